@@ -1,0 +1,142 @@
+/*
+ * muca.h -- C ABI of libmuca, the MI355X-native multicanonical Blume-Capel
+ * engine.  Plain pointers and sizes only; no C++ or GPU types cross it.
+ *
+ * The reference (null2p/MUCA) has no library API: its boundary is two CUDA
+ * programs, multigpu_weight (muca_delta_multigpu.cu) and produce_gpu
+ * (produce2_multigpu.cu), driven by positional argv and text files.  Each
+ * entry point below names the reference code it replaces; the drop-in CLIs
+ * built on it keep the reference's argv and file formats.
+ *
+ * Ownership: a context owns all device memory it allocates; host arrays
+ * belong to the caller and are only read/written during the call.
+ * Threading: contexts are independent; one context must not be used from
+ * two threads at once.  Errors: every call returns MUCA_OK (0) or a
+ * negative MUCA_E_* code; muca_last_error() gives the calling thread's
+ * last message.  Nothing in the library exits the process.
+ */
+#ifndef MUCA_H
+#define MUCA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MUCA_OK 0
+#define MUCA_E_ARG (-1)      /* invalid argument or size */
+#define MUCA_E_HIP (-2)      /* HIP runtime error */
+#define MUCA_E_NOMEM (-3)    /* device allocation failed */
+#define MUCA_E_STATE (-4)    /* call out of order (e.g. no weights set) */
+#define MUCA_E_IO (-5)       /* file could not be opened or parsed */
+
+/* acceptance precision */
+#define MUCA_REF32 0   /* float weights and diff: multigpu_weight (muca_delta_multigpu.cu:775,810) */
+#define MUCA_FP64 1    /* double weights and diff: produce_gpu (produce2_multigpu.cu:439,474) */
+
+/* Philox keying of the update and of the initial lattice */
+#define MUCA_KEYING_WEIGHT 0   /* muca_delta_multigpu.cu:340-343,740-742 */
+#define MUCA_KEYING_PRODUCE 1  /* produce2_multigpu.cu:129-130,406-409 */
+
+/* update-kernel variants (MUCA_KERNEL_AUTO lets the autotuner choose) */
+#define MUCA_KERNEL_AUTO 0
+#define MUCA_KERNEL_GENERIC 1  /* one lane per walker, reference site order from tables */
+#define MUCA_KERNEL_STRIPE 2   /* one lane per walker, even-L stripe order, register weight window */
+
+typedef struct muca_ctx muca_ctx;
+
+typedef struct muca_params {
+    uint32_t L;          /* lattice side; N = L*L spins (argv[1]) */
+    uint32_t walkers;    /* 0: maxThreadsPerCU * CUs, as muca_delta_multigpu.cu:100-106 */
+    int32_t device;      /* HIP device ordinal (first_gpu + l) */
+    uint32_t rounds;     /* Philox rounds: 7 (muca_delta_multigpu.cu:21) or 10 */
+    double beta;         /* inverse temperature; 2.0 is the reference's -dE*2 (:775) */
+    int32_t precision;   /* MUCA_REF32 | MUCA_FP64 */
+    int32_t keying;      /* MUCA_KEYING_WEIGHT | MUCA_KEYING_PRODUCE */
+    uint32_t joint_blocks; /* produce path: 0 = no joint H(D,M); B >= 1 = one joint
+                              histogram per contiguous block of walkers (jackknife) */
+    int32_t kernel;      /* MUCA_KERNEL_* */
+} muca_params;
+
+typedef struct muca_info {
+    uint32_t L, N, walkers, nbins;
+    int32_t device, kernel;
+    uint32_t cus, block;          /* compute units, threads per block of the update */
+    uint64_t lattice_bytes, joint_bins;
+} muca_info;
+
+const char *muca_last_error(void);
+int muca_default_walkers(int32_t device, uint32_t *walkers);
+
+/* Replaces the per-GPU setup of main() (muca_delta_multigpu.cu:93-205;
+ * produce2_multigpu.cu:103-275): tables, buffers, constants. */
+int muca_create(const muca_params *params, muca_ctx **ctx);
+int muca_destroy(muca_ctx *ctx);
+int muca_query(const muca_ctx *ctx, muca_info *info);
+
+/* Initial random lattice (muca_delta_multigpu.cu:340-351 for mm = jack index;
+ * produce2_multigpu.cu:126-138 ignores mm), generated on the device. */
+int muca_init_lattice(muca_ctx *ctx, uint32_t mm, uint32_t gpu_index, uint32_t seed);
+/* Lattice in the reference's layout s[site * walkers + walker]
+ * (muca_delta_multigpu.cu:151,800). */
+int muca_set_lattice(muca_ctx *ctx, const int8_t *site_major, uint64_t bytes);
+int muca_get_lattice(muca_ctx *ctx, int8_t *site_major, uint64_t bytes);
+
+/* computeEnergies/computeDeltaE (muca_delta_multigpu.cu:69-83),
+ * computeEnergy/DeltaE/M (produce2_multigpu.cu:66-89). */
+int muca_compute_state(muca_ctx *ctx);
+int muca_get_state(muca_ctx *ctx, int32_t *E, int32_t *D, int32_t *M);
+int muca_set_state(muca_ctx *ctx, const int32_t *E, const int32_t *D, const int32_t *M);
+
+/* H2D of ln W (muca_delta_multigpu.cu:386; produce2_multigpu.cu:262).  The
+ * float copy used in MUCA_REF32 mode is (float)lnw[i] (:473). */
+int muca_set_log_weight(muca_ctx *ctx, const double *lnw, uint32_t nbins);
+
+/* One launch of the update kernel `mcs` (muca_delta_multigpu.cu:393,735-829;
+ * produce2_multigpu.cu:301,311,401-496): therm attempts without and
+ * nupdates attempts with histogram accumulation, per walker.  The D
+ * histogram (nbins uint64) is zeroed first and copied to hist_out when it is
+ * non-NULL; kernel_ms (nullable) receives the update kernel's duration.
+ * In produce mode with joint_blocks > 0 the joint H(D,M) is zeroed and
+ * accumulated too (read it with muca_get_joint). */
+int muca_run(muca_ctx *ctx, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t nupdates,
+             uint64_t *hist_out, float *kernel_ms);
+/* joint histogram of one walker block: (N+1)(N+2)/2 bins at (M+D*D+2*D)/2 */
+int muca_get_joint(muca_ctx *ctx, uint32_t block, uint64_t *joint, uint64_t nbins);
+
+/* ---- host-side weight iteration pieces (O(N), run between launches) ---- */
+/* hist_range (muca_delta_multigpu.cu:573-596) */
+int muca_hist_range(const uint64_t *hist, uint32_t N, int32_t *start, int32_t *end);
+/* kull_leibler (muca_delta_multigpu.cu:598-618) */
+double muca_kull_leibler(const uint64_t *hist, uint32_t N);
+/* trivial update, lnW[i] -= log(H[i]) for H[i] > 1 (:467) */
+int muca_update_trivial(const uint64_t *hist, uint32_t N, double *lnw);
+/* weight_recursion (:833-859); p_acc persists across iterations of one run */
+int muca_update_recursive(const uint64_t *hist, uint32_t N, int32_t start, int32_t end,
+                          double *p_acc, double *lnw);
+/* lnW -= lnW[0] over all bins (:469-475) */
+int muca_normalise(uint32_t N, double *lnw);
+/* jakknife (:620-635; produce2_multigpu.cu:520-535) */
+int muca_jackknife(const double *x, uint32_t J, double *avg, double *err);
+
+/* ---- produce path on the device ---- */
+/* Reweighting of the joint histograms to canonical <|M|>, chi and the Binder
+ * cumulant at ndelta crystal fields (produce2_multigpu.cu:293-358,391-400,
+ * 498-518), in FP64 on the device with a max-shifted log-sum-exp.  Uses the
+ * ln W last set with muca_set_log_weight.  Outputs are [ndelta][blocks]. */
+int muca_reweight(muca_ctx *ctx, const double *deltas, uint32_t ndelta, double *absM,
+                  double *chi, double *binder);
+
+/* ---- self-checks usable without a GPU ---- */
+/* the acceptance log evaluated by the host instantiation of the device code */
+float muca_acclog_host(uint32_t r);
+/* host evaluation for r = k << 9, k in [k0, k0+n) */
+int muca_acclog_host_range(uint32_t k0, uint32_t n, float *out);
+/* the same function evaluated on the device for r = k << 9, k in [k0, k0+n) */
+int muca_acclog_device(int32_t device, uint32_t k0, uint32_t n, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,292 @@
+"""muca_amd -- MI355X-native multicanonical Blume-Capel engine.
+
+Python view of libmuca.so (include/muca.h) through ctypes.  The library
+must be built in-tree first (``python -m muca_amd.build`` or
+``__graft_entry__.build()``); importing any engine entry point without it
+raises -- there is no CPU fallback.
+
+The reference's own boundary is its two CLIs (muca_delta_multigpu.cu,
+produce2_multigpu.cu); muca_amd.bin holds drop-in builds of them.  This module
+exposes the same steps to Python: Context (one device, one walker set),
+run() for the `mcs` kernel, and the O(N) host pieces of the weight loop.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmuca.so")
+BIN_DIR = os.path.join(PKG, "bin")
+
+REF32, FP64 = 0, 1
+KEYING_WEIGHT, KEYING_PRODUCE = 0, 1
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_STRIPE = 0, 1, 2
+SEED = 100                       # muca_delta_multigpu.cu:19
+
+_u8p = ctypes.POINTER(ctypes.c_int8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class MucaError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("walkers", ctypes.c_uint32),
+                ("device", ctypes.c_int32), ("rounds", ctypes.c_uint32),
+                ("beta", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("keying", ctypes.c_int32), ("joint_blocks", ctypes.c_uint32),
+                ("kernel", ctypes.c_int32)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("N", ctypes.c_uint32), ("walkers", ctypes.c_uint32),
+                ("nbins", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("kernel", ctypes.c_int32), ("cus", ctypes.c_uint32),
+                ("block", ctypes.c_uint32), ("lattice_bytes", ctypes.c_uint64),
+                ("joint_bins", ctypes.c_uint64)]
+
+
+# every symbol declared in include/muca.h, with its ctypes signature
+_SIGS = {
+    "muca_last_error": (ctypes.c_char_p, []),
+    "muca_default_walkers": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32)]),
+    "muca_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    "muca_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "muca_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Info)]),
+    "muca_init_lattice": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32]),
+    "muca_set_lattice": (ctypes.c_int, [ctypes.c_void_p, _u8p, ctypes.c_uint64]),
+    "muca_get_lattice": (ctypes.c_int, [ctypes.c_void_p, _u8p, ctypes.c_uint64]),
+    "muca_compute_state": (ctypes.c_int, [ctypes.c_void_p]),
+    "muca_get_state": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
+    "muca_set_state": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
+    "muca_set_log_weight": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_uint32]),
+    "muca_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint64, ctypes.c_uint64, _u64p, _f32p]),
+    "muca_get_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
+                                      ctypes.c_uint64]),
+    "muca_hist_range": (ctypes.c_int, [_u64p, ctypes.c_uint32, _i32p, _i32p]),
+    "muca_kull_leibler": (ctypes.c_double, [_u64p, ctypes.c_uint32]),
+    "muca_update_trivial": (ctypes.c_int, [_u64p, ctypes.c_uint32, _f64p]),
+    "muca_update_recursive": (ctypes.c_int, [_u64p, ctypes.c_uint32, ctypes.c_int32,
+                                             ctypes.c_int32, _f64p, _f64p]),
+    "muca_normalise": (ctypes.c_int, [ctypes.c_uint32, _f64p]),
+    "muca_jackknife": (ctypes.c_int, [_f64p, ctypes.c_uint32, _f64p, _f64p]),
+    "muca_reweight": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_uint32, _f64p, _f64p,
+                                     _f64p]),
+    "muca_acclog_host": (ctypes.c_float, [ctypes.c_uint32]),
+    "muca_acclog_host_range": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _f32p]),
+    "muca_acclog_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32,
+                                          _f32p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmuca.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MucaError(f"{LIB_PATH} is not built; run `python -m muca_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MucaError(f"libmuca error {rc}: {lib().muca_last_error().decode()}")
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_walkers(device=0):
+    w = ctypes.c_uint32()
+    _check(lib().muca_default_walkers(device, ctypes.byref(w)))
+    return w.value
+
+
+class Context:
+    """One device and one set of walkers (the per-GPU state of the reference's
+    main(), muca_delta_multigpu.cu:177-205 / produce2_multigpu.cu:247-275)."""
+
+    def __init__(self, L, walkers=0, device=0, *, rounds=7, beta=2.0, precision=REF32,
+                 keying=KEYING_WEIGHT, joint_blocks=0, kernel=KERNEL_AUTO):
+        self._h = ctypes.c_void_p()
+        p = Params(L, walkers, device, rounds, beta, precision, keying, joint_blocks, kernel)
+        _check(lib().muca_create(ctypes.byref(p), ctypes.byref(self._h)))
+        info = Info()
+        _check(lib().muca_query(self._h, ctypes.byref(info)))
+        self.info = info
+        self.L, self.N, self.W = info.L, info.N, info.walkers
+        self.nbins = info.nbins
+        self.joint_blocks = joint_blocks
+
+    def close(self):
+        if self._h:
+            lib().muca_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init_lattice(self, mm=0, gpu=0, seed=SEED):
+        _check(lib().muca_init_lattice(self._h, mm, gpu, seed))
+
+    def set_lattice(self, site_major):
+        a = np.ascontiguousarray(site_major, dtype=np.int8)
+        _check(lib().muca_set_lattice(self._h, _p(a, _u8p), a.size))
+
+    def get_lattice(self):
+        """Site-major lattice, shape (N, walkers)."""
+        a = np.empty((self.N, self.W), np.int8)
+        _check(lib().muca_get_lattice(self._h, _p(a, _u8p), a.size))
+        return a
+
+    def compute_state(self):
+        _check(lib().muca_compute_state(self._h))
+
+    def get_state(self):
+        E, D, M = (np.empty(self.W, np.int32) for _ in range(3))
+        _check(lib().muca_get_state(self._h, _p(E, _i32p), _p(D, _i32p), _p(M, _i32p)))
+        return E, D, M
+
+    def set_state(self, E=None, D=None, M=None):
+        a = [None if x is None else np.ascontiguousarray(x, np.int32) for x in (E, D, M)]
+        _check(lib().muca_set_state(self._h, *[_p(x, _i32p) for x in a]))
+
+    def set_log_weight(self, lnw):
+        a = np.ascontiguousarray(lnw, np.float64)
+        _check(lib().muca_set_log_weight(self._h, _p(a, _f64p), a.size))
+
+    def run(self, iter, seed, therm, nupdates, want_hist=True):
+        """One update launch; returns (hist uint64[N+1] or None, kernel ms)."""
+        h = np.empty(self.nbins, np.uint64) if want_hist else None
+        ms = ctypes.c_float()
+        _check(lib().muca_run(self._h, iter & 0xFFFFFFFF, seed, therm, nupdates, _p(h, _u64p),
+                              ctypes.byref(ms)))
+        return h, ms.value
+
+    def get_joint(self, block=0):
+        n = (self.N + 1) * (self.N + 2) // 2
+        a = np.empty(n, np.uint64)
+        _check(lib().muca_get_joint(self._h, block, _p(a, _u64p), n))
+        return a
+
+    def reweight(self, deltas):
+        """(<|M|>, chi, Binder) arrays of shape (len(deltas), joint_blocks)."""
+        d = np.ascontiguousarray(deltas, np.float64)
+        shp = (d.size, self.joint_blocks)
+        out = [np.empty(shp, np.float64) for _ in range(3)]
+        _check(lib().muca_reweight(self._h, _p(d, _f64p), d.size, *[_p(o, _f64p) for o in out]))
+        return out
+
+
+# ---- host pieces of the weight iteration (muca_delta_multigpu.cu:431-476) ----
+def hist_range(hist):
+    h = np.ascontiguousarray(hist, np.uint64)
+    s, e = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().muca_hist_range(_p(h, _u64p), h.size - 1, ctypes.byref(s), ctypes.byref(e)))
+    return s.value, e.value
+
+
+def kull_leibler(hist):
+    h = np.ascontiguousarray(hist, np.uint64)
+    return lib().muca_kull_leibler(_p(h, _u64p), h.size - 1)
+
+
+def update_trivial(hist, lnw):
+    h = np.ascontiguousarray(hist, np.uint64)
+    _check(lib().muca_update_trivial(_p(h, _u64p), h.size - 1, _p(lnw, _f64p)))
+
+
+def update_recursive(hist, start, end, p_acc, lnw):
+    h = np.ascontiguousarray(hist, np.uint64)
+    _check(lib().muca_update_recursive(_p(h, _u64p), h.size - 1, start, end, _p(p_acc, _f64p),
+                                       _p(lnw, _f64p)))
+
+
+def normalise(lnw):
+    _check(lib().muca_normalise(lnw.size - 1, _p(lnw, _f64p)))
+
+
+def jackknife(x):
+    a = np.ascontiguousarray(x, np.float64)
+    avg, err = ctypes.c_double(), ctypes.c_double()
+    _check(lib().muca_jackknife(_p(a, _f64p), a.size, ctypes.byref(avg), ctypes.byref(err)))
+    return avg.value, err.value
+
+
+def acclog_host(r):
+    return lib().muca_acclog_host(r)
+
+
+def acclog_host_range(k0, n):
+    out = np.empty(n, np.float32)
+    _check(lib().muca_acclog_host_range(k0, n, _p(out, _f32p)))
+    return out
+
+
+def acclog_device(k0, n, device=0):
+    out = np.empty(n, np.float32)
+    _check(lib().muca_acclog_device(device, k0, n, _p(out, _f32p)))
+    return out
+
+
+def weight_iteration(ctx, iters, *, seed=SEED, init_nup=1, recursion=False, lnw_init=None,
+                     callback=None, max_converged_stop=True):
+    """The weight loop of multigpu_weight (muca_delta_multigpu.cu:374-477) on one
+    device: update launch with iter=k, width/D_KL bookkeeping, NUPDATES
+    schedule, trivial or recursive update, float copy.  Returns the final
+    double ln W and a list of per-iteration dicts."""
+    N = ctx.N
+    lnw = np.zeros(N + 1) if lnw_init is None else np.array(lnw_init, np.float64)
+    ctx.set_log_weight(lnw)
+    p_acc = np.zeros(N + 1)
+    width, nup, recs = 10, init_nup, []
+    for k in range(iters):
+        if width == N:
+            nup = int((nup + 1) * 1.01)
+        hist, ms = ctx.run(k, seed, N, nup)
+        s, e = hist_range(hist)
+        wt = (e - s) & 0xFFFFFFFF
+        width = max(width, wt)
+        dkl = kull_leibler(hist)
+        conv = width == N and dkl < 1e-4
+        rec = dict(k=k, hist=hist, width_temp=wt, width=width, dkl=dkl, nup=nup,
+                   kernel_ms=ms, converged=conv)
+        if not conv:
+            if recursion:
+                update_recursive(hist, s, e, p_acc, lnw)
+            else:
+                update_trivial(hist, lnw)
+            normalise(lnw)
+            ctx.set_log_weight(lnw)
+        rec["lnw"] = lnw.copy()
+        recs.append(rec)
+        if callback:
+            callback(rec)
+        if conv and max_converged_stop:
+            break
+    return lnw, recs

@@ -1,0 +1,86 @@
+"""Build libmuca.so and the drop-in CLIs in-tree, for gfx950 only.
+
+    python -m muca_amd.build          # or muca_amd.build.build()
+
+Outputs (git-ignored, shipped with the tree):
+    muca_amd/libmuca.so               the C-ABI library (include/muca.h)
+    muca_amd/bin/multigpu_weight      drop-in for muca_delta_multigpu.cu's binary
+    muca_amd/bin/produce_gpu          drop-in for produce2_multigpu.cu's binary
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BIN = os.path.join(PKG, "bin")
+LIB = os.path.join(PKG, "libmuca.so")
+
+ARCH = "gfx950"
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# -ffp-contract=off: every float/double expression of the acceptance rounds
+# in the order it is written (fma only where written explicitly).
+# -packed-fp32-ops off: no v_pk_*_f32 in generated code.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
+
+LIB_SOURCES = ["muca_kernels.hip", "muca_api.cpp"]
+CLI_SOURCES = {"multigpu_weight": "cli_weight.cpp", "produce_gpu": "cli_produce.cpp"}
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout + p.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return p
+
+
+def build_lib(verbose=False):
+    objs = []
+    os.makedirs(os.path.join(PKG, "_build"), exist_ok=True)
+    for src in LIB_SOURCES:
+        obj = os.path.join(PKG, "_build", src + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
+               os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd[1:1] = ["-x", "hip"]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_cli(verbose=False):
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for name, src in CLI_SOURCES.items():
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        exe = os.path.join(BIN, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), path,
+               "-o", exe, "-L", PKG, "-lmuca", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        out.append(exe)
+    return out
+
+
+def build(verbose=False):
+    lib = build_lib(verbose)
+    clis = build_cli(verbose)
+    return [lib] + clis
+
+
+if __name__ == "__main__":
+    for p in build(verbose="-v" in sys.argv):
+        print(p)

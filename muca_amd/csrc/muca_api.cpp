@@ -1,0 +1,528 @@
+// muca_api.cpp -- the C ABI of libmuca (include/muca.h): contexts, device
+// buffers, launches and the O(N) host-side pieces of the weight iteration.
+#include "../../include/muca.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "acclog.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(MUCA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct muca_ctx {
+    muca_params p{};
+    uint32_t N = 0;
+    uint64_t W = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int8_t *lat = nullptr;
+    int32_t *E = nullptr, *D = nullptr, *M = nullptr;
+    float *lnw32 = nullptr;
+    double *lnw64 = nullptr;
+    uint32_t *order = nullptr, *naver = nullptr;
+    uint64_t *hist = nullptr;
+    uint64_t *joint = nullptr;
+    uint64_t joint_stride = 0, block_walkers = 0;
+    double *mom = nullptr, *deltas = nullptr, *rw_out = nullptr;
+    uint32_t rw_cap = 0;
+    bool have_weights = false;
+    muca::Launch cfg{};
+    uint32_t cus = 0;
+};
+
+namespace {
+
+// neighbour and order tables exactly as muca_delta_multigpu.cu:157-175
+void make_tables(uint32_t L, std::vector<uint32_t> &naver, std::vector<uint32_t> &order)
+{
+    const uint32_t N = L * L;
+    naver.assign(4ull * N, 0);
+    order.assign(N, 0);
+    for (uint32_t i = 0; i < N; i++) {
+        int right = (int)i + 1, left = (int)i - 1, up = (int)(i + L), down = (int)i - (int)L;
+        if (right % (int)L == 0) right -= (int)L;
+        if (i % L == 0) left += (int)L;
+        if (up > (int)N - 1) up -= (int)N;
+        if (down < 0) down += (int)N;
+        naver[4ull * i + 0] = (uint32_t)right;
+        naver[4ull * i + 1] = (uint32_t)left;
+        naver[4ull * i + 2] = (uint32_t)up;
+        naver[4ull * i + 3] = (uint32_t)down;
+        const uint32_t half = N / 2;
+        const uint32_t blk = half ? (i / half) % 2 : 0;
+        order[i] = ((0 == blk) * (2 * i)) % N + ((1 == blk) * (2 * i + 1)) % N;
+    }
+}
+
+void free_ctx(muca_ctx *c)
+{
+    if (!c) return;
+    if (c->p.device >= 0) hipSetDevice(c->p.device);
+    void *bufs[] = {c->lat, c->E, c->D, c->M, c->lnw32, c->lnw64, c->order, c->naver,
+                    c->hist, c->joint, c->mom, c->deltas, c->rw_out};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+template <typename T>
+int dev_alloc(T **p, uint64_t count, const char *what)
+{
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T) ? count * sizeof(T) : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(MUCA_E_NOMEM, std::string("hipMalloc(") + what + ", " +
+                                      std::to_string(count * sizeof(T)) +
+                                      " bytes): " + hipGetErrorString(e));
+    }
+    return MUCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *muca_last_error(void)
+{
+    return g_err.c_str();
+}
+
+int muca_default_walkers(int32_t device, uint32_t *walkers)
+{
+    if (!walkers) return fail(MUCA_E_ARG, "walkers is NULL");
+    int mt = 0, cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&mt, hipDeviceAttributeMaxThreadsPerMultiProcessor, device));
+    HIP_TRY(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+    *walkers = (uint32_t)mt * (uint32_t)cu;
+    return MUCA_OK;
+}
+
+int muca_create(const muca_params *params, muca_ctx **out)
+{
+    if (!params || !out) return fail(MUCA_E_ARG, "NULL argument");
+    *out = nullptr;
+    const muca_params &p = *params;
+    if (p.L < 2 || p.L > 46340) return fail(MUCA_E_ARG, "L must be in [2, 46340]");
+    if (p.rounds != 7 && p.rounds != 10) return fail(MUCA_E_ARG, "rounds must be 7 or 10");
+    if (p.precision != MUCA_REF32 && p.precision != MUCA_FP64)
+        return fail(MUCA_E_ARG, "precision must be MUCA_REF32 or MUCA_FP64");
+    if (p.keying != MUCA_KEYING_WEIGHT && p.keying != MUCA_KEYING_PRODUCE)
+        return fail(MUCA_E_ARG, "keying must be MUCA_KEYING_WEIGHT or MUCA_KEYING_PRODUCE");
+    if (!(p.beta > 0) || !std::isfinite(p.beta)) return fail(MUCA_E_ARG, "beta must be > 0");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (p.device < 0 || p.device >= ndev)
+        return fail(MUCA_E_ARG, "device " + std::to_string(p.device) + " not present (" +
+                                    std::to_string(ndev) + " visible)");
+    HIP_TRY(hipSetDevice(p.device));
+    muca_ctx *c = new muca_ctx();
+    c->p = p;
+    c->N = p.L * p.L;
+    uint32_t W = p.walkers;
+    if (W == 0) {
+        int rc = muca_default_walkers(p.device, &W);
+        if (rc) {
+            free_ctx(c);
+            return rc;
+        }
+    }
+    c->W = W;
+    c->p.walkers = W;
+    int cu = 0, lds = 0;
+    hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, p.device);
+    hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p.device);
+    c->cus = (uint32_t)cu;
+    c->cfg.rounds = (int)p.rounds;
+    c->cfg.fp64 = p.precision == MUCA_FP64;
+    c->cfg.keying = p.keying;
+    c->cfg.kernel = p.kernel == MUCA_KERNEL_AUTO ? MUCA_KERNEL_GENERIC : p.kernel;
+    c->cfg.block = 256;
+    c->cfg.lds_max = (size_t)(lds > 0 ? lds : 65536);
+    const uint64_t N = c->N;
+    int rc = MUCA_OK;
+    if (!rc) rc = dev_alloc(&c->lat, N * W, "lattice");
+    if (!rc) rc = dev_alloc(&c->E, W, "E");
+    if (!rc) rc = dev_alloc(&c->D, W, "D");
+    if (!rc) rc = dev_alloc(&c->M, W, "M");
+    if (!rc) rc = dev_alloc(&c->lnw32, N + 1, "lnw32");
+    if (!rc) rc = dev_alloc(&c->lnw64, N + 1, "lnw64");
+    if (!rc) rc = dev_alloc(&c->order, N, "order");
+    if (!rc) rc = dev_alloc(&c->naver, 4 * N, "naver");
+    if (!rc) rc = dev_alloc(&c->hist, N + 1, "hist");
+    if (!rc && p.joint_blocks > 0) {
+        if (p.joint_blocks > W) {
+            free_ctx(c);
+            return fail(MUCA_E_ARG, "joint_blocks exceeds walkers");
+        }
+        c->joint_stride = (N + 1) * (N + 2) / 2;
+        c->block_walkers = (W + p.joint_blocks - 1) / p.joint_blocks;
+        rc = dev_alloc(&c->joint, c->joint_stride * p.joint_blocks, "joint histogram");
+    }
+    if (rc) {
+        free_ctx(c);
+        return rc;
+    }
+    std::vector<uint32_t> naver, order;
+    make_tables(p.L, naver, order);
+    hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->naver, naver.data(), naver.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->order, order.data(), order.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->lat, 0, N * W);
+    if (e == hipSuccess) e = hipMemset(c->E, 0, W * 4);
+    if (e == hipSuccess) e = hipMemset(c->D, 0, W * 4);
+    if (e == hipSuccess) e = hipMemset(c->M, 0, W * 4);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(MUCA_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return MUCA_OK;
+}
+
+int muca_destroy(muca_ctx *c)
+{
+    free_ctx(c);
+    return MUCA_OK;
+}
+
+int muca_query(const muca_ctx *c, muca_info *info)
+{
+    if (!c || !info) return fail(MUCA_E_ARG, "NULL argument");
+    info->L = c->p.L;
+    info->N = c->N;
+    info->walkers = (uint32_t)c->W;
+    info->nbins = c->N + 1;
+    info->device = c->p.device;
+    info->kernel = c->cfg.kernel;
+    info->cus = c->cus;
+    info->block = (uint32_t)c->cfg.block;
+    info->lattice_bytes = (uint64_t)c->N * c->W;
+    info->joint_bins = c->joint_stride * c->p.joint_blocks;
+    return MUCA_OK;
+}
+
+int muca_init_lattice(muca_ctx *c, uint32_t mm, uint32_t gpu_index, uint32_t seed)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(muca::launch_init_lattice(c->st, c->lat, c->W, c->N, c->p.keying, (int)c->p.rounds,
+                                      mm, gpu_index, seed, 0));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return MUCA_OK;
+}
+
+int muca_set_lattice(muca_ctx *c, const int8_t *src, uint64_t bytes)
+{
+    if (!c || !src) return fail(MUCA_E_ARG, "NULL argument");
+    if (bytes != (uint64_t)c->N * c->W) return fail(MUCA_E_ARG, "lattice size mismatch");
+    for (uint64_t i = 0; i < bytes; i++)
+        if (src[i] < -1 || src[i] > 1) return fail(MUCA_E_ARG, "spins must be in {-1,0,1}");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipMemcpy(c->lat, src, bytes, hipMemcpyHostToDevice));
+    return MUCA_OK;
+}
+
+int muca_get_lattice(muca_ctx *c, int8_t *dst, uint64_t bytes)
+{
+    if (!c || !dst) return fail(MUCA_E_ARG, "NULL argument");
+    if (bytes != (uint64_t)c->N * c->W) return fail(MUCA_E_ARG, "lattice size mismatch");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(dst, c->lat, bytes, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_compute_state(muca_ctx *c)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(muca::launch_state(c->st, c->lat, c->W, c->N, c->naver, c->E, c->D, c->M));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return MUCA_OK;
+}
+
+int muca_get_state(muca_ctx *c, int32_t *E, int32_t *D, int32_t *M)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (E) HIP_TRY(hipMemcpy(E, c->E, c->W * 4, hipMemcpyDeviceToHost));
+    if (D) HIP_TRY(hipMemcpy(D, c->D, c->W * 4, hipMemcpyDeviceToHost));
+    if (M) HIP_TRY(hipMemcpy(M, c->M, c->W * 4, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_set_state(muca_ctx *c, const int32_t *E, const int32_t *D, const int32_t *M)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    if (D)
+        for (uint64_t w = 0; w < c->W; w++)
+            if (D[w] < 0 || (uint32_t)D[w] > c->N) return fail(MUCA_E_ARG, "D out of [0, N]");
+    HIP_TRY(hipSetDevice(c->p.device));
+    if (E) HIP_TRY(hipMemcpy(c->E, E, c->W * 4, hipMemcpyHostToDevice));
+    if (D) HIP_TRY(hipMemcpy(c->D, D, c->W * 4, hipMemcpyHostToDevice));
+    if (M) HIP_TRY(hipMemcpy(c->M, M, c->W * 4, hipMemcpyHostToDevice));
+    return MUCA_OK;
+}
+
+int muca_set_log_weight(muca_ctx *c, const double *lnw, uint32_t nbins)
+{
+    if (!c || !lnw) return fail(MUCA_E_ARG, "NULL argument");
+    if (nbins != c->N + 1) return fail(MUCA_E_ARG, "nbins must be N+1");
+    std::vector<float> f(nbins);
+    for (uint32_t i = 0; i < nbins; i++) {
+        if (!std::isfinite(lnw[i])) return fail(MUCA_E_ARG, "ln W must be finite");
+        f[i] = (float)lnw[i];
+    }
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipMemcpy(c->lnw64, lnw, nbins * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->lnw32, f.data(), nbins * sizeof(float), hipMemcpyHostToDevice));
+    c->have_weights = true;
+    return MUCA_OK;
+}
+
+int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t nup,
+             uint64_t *hist_out, float *kernel_ms)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    if (!c->have_weights) return fail(MUCA_E_STATE, "muca_set_log_weight was not called");
+    HIP_TRY(hipSetDevice(c->p.device));
+    const uint64_t N = c->N;
+    HIP_TRY(hipMemsetAsync(c->hist, 0, (N + 1) * sizeof(uint64_t), c->st));
+    const bool joint = c->joint && c->p.precision == MUCA_FP64 && nup > 0;
+    if (joint)
+        HIP_TRY(hipMemsetAsync(c->joint, 0, c->joint_stride * c->p.joint_blocks * 8, c->st));
+    muca::McsArgs a{};
+    a.lat = c->lat;
+    a.E = c->E;
+    a.D = c->D;
+    a.M = c->M;
+    a.lnw32 = c->lnw32;
+    a.lnw64 = c->lnw64;
+    a.order = c->order;
+    a.naver = c->naver;
+    a.hist = c->hist;
+    a.joint = joint ? c->joint : nullptr;
+    a.W = c->W;
+    a.N = c->N;
+    a.L = c->p.L;
+    a.w_offset = 0;
+    a.iter = iter;
+    a.seed = seed;
+    a.therm = therm;
+    a.nup = nup;
+    a.beta = c->p.beta;
+    a.joint_stride = c->joint_stride;
+    a.block_walkers = c->block_walkers ? c->block_walkers : c->W;
+    a.joint_blocks = c->p.joint_blocks;
+    HIP_TRY(hipEventRecord(c->ev0, c->st));
+    HIP_TRY(muca::launch_mcs(c->st, c->cfg, a));
+    HIP_TRY(hipEventRecord(c->ev1, c->st));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, c->ev0, c->ev1));
+    if (hist_out)
+        HIP_TRY(hipMemcpy(hist_out, c->hist, (N + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_get_joint(muca_ctx *c, uint32_t block, uint64_t *dst, uint64_t nbins)
+{
+    if (!c || !dst) return fail(MUCA_E_ARG, "NULL argument");
+    if (!c->joint) return fail(MUCA_E_STATE, "context has no joint histogram");
+    if (block >= c->p.joint_blocks) return fail(MUCA_E_ARG, "block out of range");
+    if (nbins != c->joint_stride) return fail(MUCA_E_ARG, "nbins must be (N+1)(N+2)/2");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(dst, c->joint + (uint64_t)block * c->joint_stride, nbins * 8,
+                      hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_reweight(muca_ctx *c, const double *deltas, uint32_t ndelta, double *absM,
+                  double *chi, double *binder)
+{
+    if (!c || !deltas || ndelta == 0) return fail(MUCA_E_ARG, "bad arguments");
+    if (!c->joint) return fail(MUCA_E_STATE, "context has no joint histogram");
+    if (!c->have_weights) return fail(MUCA_E_STATE, "muca_set_log_weight was not called");
+    HIP_TRY(hipSetDevice(c->p.device));
+    const uint32_t B = c->p.joint_blocks;
+    const uint64_t nb = c->N + 1;
+    if (!c->mom) {
+        int rc = dev_alloc(&c->mom, (uint64_t)B * 4 * nb, "moments");
+        if (rc) return rc;
+    }
+    if (ndelta > c->rw_cap) {
+        if (c->deltas) hipFree(c->deltas);
+        if (c->rw_out) hipFree(c->rw_out);
+        c->deltas = nullptr;
+        c->rw_out = nullptr;
+        c->rw_cap = 0;
+        int rc = dev_alloc(&c->deltas, ndelta, "deltas");
+        if (!rc) rc = dev_alloc(&c->rw_out, (uint64_t)ndelta * B * 3, "reweight output");
+        if (rc) return rc;
+        c->rw_cap = ndelta;
+    }
+    HIP_TRY(hipMemcpyAsync(c->deltas, deltas, ndelta * 8, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(muca::launch_joint_moments(c->st, c->joint, c->N, c->joint_stride, B, c->mom));
+    HIP_TRY(muca::launch_reweight(c->st, c->mom, c->lnw64, c->N, B, c->p.beta, c->deltas,
+                                  ndelta, c->rw_out));
+    std::vector<double> o((size_t)ndelta * B * 3);
+    HIP_TRY(hipMemcpyAsync(o.data(), c->rw_out, o.size() * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < (size_t)ndelta * B; i++) {
+        if (absM) absM[i] = o[3 * i];
+        if (chi) chi[i] = o[3 * i + 1];
+        if (binder) binder[i] = o[3 * i + 2];
+    }
+    return MUCA_OK;
+}
+
+// ---- host-side pieces of the weight iteration ------------------------------
+
+int muca_hist_range(const uint64_t *hist, uint32_t N, int32_t *start, int32_t *end)
+{
+    if (!hist || !start || !end) return fail(MUCA_E_ARG, "NULL argument");
+    int32_t s = (int32_t)N + 1, e = 0;
+    for (uint32_t i = 0; i <= N; i++)
+        if (hist[i] > 1) {
+            s = (int32_t)i;
+            break;
+        }
+    for (uint32_t i = N + 1; i-- > 0;)
+        if (hist[i] > 1) {
+            e = (int32_t)i;
+            break;
+        }
+    if (s > e) s = e = 0;
+    *start = s;
+    *end = e;
+    return MUCA_OK;
+}
+
+double muca_kull_leibler(const uint64_t *hist, uint32_t N)
+{
+    double bins = 0, events = 0, kld = 0;
+    for (uint32_t i = 0; i <= N; i++)
+        if (hist[i] > 0) {
+            events = events + (double)hist[i];
+            bins = bins + 1.0;
+        }
+    for (uint32_t i = 0; i <= N; i++)
+        if (hist[i] > 0) {
+            const double P = (double)hist[i] / events;
+            const double Q = 1.0 / bins;
+            kld = kld + P * std::log(P / Q);
+        }
+    return kld;
+}
+
+int muca_update_trivial(const uint64_t *hist, uint32_t N, double *lnw)
+{
+    if (!hist || !lnw) return fail(MUCA_E_ARG, "NULL argument");
+    for (uint32_t i = 0; i <= N; i++)
+        if (hist[i] > 1) lnw[i] = lnw[i] - std::log((double)hist[i]);
+    return MUCA_OK;
+}
+
+int muca_update_recursive(const uint64_t *hist, uint32_t N, int32_t start, int32_t end,
+                          double *p_acc, double *lnw)
+{
+    if (!hist || !lnw || !p_acc) return fail(MUCA_E_ARG, "NULL argument");
+    if (start < 0 || end > (int32_t)N || start > end) return fail(MUCA_E_ARG, "bad range");
+    std::vector<double> bf(lnw, lnw + N + 1);
+    for (int32_t i = start; i < end; i++) {
+        const uint64_t a = hist[i], b = hist[i + 1];
+        if ((a != 0) & (b != 0)) {
+            const double p = (double)(a * b) / (double)(a + b);
+            const double k = p / (p + p_acc[i]);
+            const double dS = -bf[i] + bf[i + 1] + k * std::log((double)a / (double)b);
+            lnw[i + 1] = lnw[i] + dS;
+            p_acc[i] = p_acc[i] + p;
+        }
+    }
+    return MUCA_OK;
+}
+
+int muca_normalise(uint32_t N, double *lnw)
+{
+    if (!lnw) return fail(MUCA_E_ARG, "NULL argument");
+    const double t = lnw[0];
+    for (uint32_t i = 0; i <= N; i++) lnw[i] = lnw[i] - t;
+    return MUCA_OK;
+}
+
+int muca_jackknife(const double *x, uint32_t J, double *avg_out, double *err_out)
+{
+    if (!x || !avg_out || !err_out || J == 0) return fail(MUCA_E_ARG, "bad arguments");
+    double avg = 0;
+    for (uint32_t i = 0; i < J; i++) avg += x[i];
+    std::vector<double> mi(J);
+    for (uint32_t i = 0; i < J; i++) mi[i] = (avg - x[i]) / (double)(J - 1);
+    avg = avg / (double)J;
+    double dev = 0;
+    for (uint32_t i = 0; i < J; i++) dev += (avg - mi[i]) * (avg - mi[i]);
+    dev = (J - 1) * dev / (double)J;
+    *avg_out = avg;
+    *err_out = std::sqrt(dev);
+    return MUCA_OK;
+}
+
+// ---- self-checks -----------------------------------------------------------
+
+float muca_acclog_host(uint32_t r)
+{
+    return acclog_u01(r);
+}
+
+int muca_acclog_host_range(uint32_t k0, uint32_t n, float *out)
+{
+    if (!out) return fail(MUCA_E_ARG, "NULL argument");
+    for (uint32_t t = 0; t < n; t++) out[t] = acclog_u01((k0 + t) << 9);
+    return MUCA_OK;
+}
+
+int muca_acclog_device(int32_t device, uint32_t k0, uint32_t n, float *out)
+{
+    if (!out || n == 0) return fail(MUCA_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(device));
+    float *d = nullptr;
+    int rc = dev_alloc(&d, n, "acclog");
+    if (rc) return rc;
+    hipError_t e = muca::launch_acclog(nullptr, k0, n, d);
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * 4, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MUCA_E_HIP, std::string("acclog: ") + hipGetErrorString(e));
+    return MUCA_OK;
+}
+
+}  // extern "C"

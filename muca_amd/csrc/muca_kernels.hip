@@ -1,0 +1,402 @@
+// muca_kernels.hip -- HIP kernels of libmuca for gfx950 (MI355X).
+//
+// Update-kernel design: every walker is a strict serial Metropolis chain
+// (fixed site order, acceptance depends on the running D = sum s^2), so the
+// parallelism is across walkers: one lane per walker, walkers interleaved
+// site-major in HBM so that each site access of a wavefront is one
+// contiguous 64-byte segment.  The D histogram is privatised per workgroup
+// in LDS (uint32) and flushed to HBM with 64-bit atomics.  Nothing in the
+// update is a dense contraction, so there is no MFMA here.
+#include "kernels.h"
+
+#include "acclog.h"
+#include "philox.h"
+
+namespace muca {
+
+#define MUCA_CTR3 0xBADC0DEDu
+
+// ---------------------------------------------------------------------------
+// initial lattice (muca_delta_multigpu.cu:340-351; produce2_multigpu.cu:126-138)
+// one thread per (walker, block of four sites); stores are coalesced over walkers
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_init_lattice(int8_t *lat, uint64_t W, uint32_t N,
+                                                      int keying, uint32_t mm, uint32_t gpu,
+                                                      uint32_t seed, uint32_t w_offset)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nblk = (N + 3) / 4;
+    if (t >= W * nblk) return;
+    const uint64_t w = t % W;
+    const uint32_t blk = (uint32_t)(t / W);
+    const uint32_t wid = w_offset + (uint32_t)w;
+    philox4 r;
+    if (keying == 0)
+        r = philox4x32<R>(gpu * 2u + blk + 1u, seed, 0xBADCAB1Eu, 0xBADC0DEDu,
+                          10u * wid + mm * 2u, 0xdecafbadu);
+    else
+        r = philox4x32<R>(gpu * 1000u, seed, blk + 1u, 0xBADCAFE3u, wid * 10u, 0xdecafbadu);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t site = 4u * blk + (uint32_t)j;
+        if (site < N) {
+            const double u = (double)((float)(1u | (r.v[j] >> 8)) * 0x1p-24f);
+            lat[(uint64_t)site * W + w] = (int8_t)((u < 1 / 3.0) + (u < 2 / 3.0) - 1);
+        }
+    }
+}
+
+hipError_t launch_init_lattice(hipStream_t st, int8_t *lat, uint64_t W, uint32_t N,
+                               int keying, int rounds, uint32_t mm, uint32_t gpu,
+                               uint32_t seed, uint32_t w_offset)
+{
+    const uint64_t total = W * ((N + 3) / 4);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (rounds == 10)
+        hipLaunchKernelGGL(k_init_lattice<10>, grid, dim3(256), 0, st, lat, W, N, keying, mm,
+                           gpu, seed, w_offset);
+    else
+        hipLaunchKernelGGL(k_init_lattice<7>, grid, dim3(256), 0, st, lat, W, N, keying, mm,
+                           gpu, seed, w_offset);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// per-walker E_J, D, M (muca_delta_multigpu.cu:69-83,548-571; produce2_multigpu.cu:66-89)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_state(const int8_t *lat, uint64_t W, uint32_t N,
+                                               const uint32_t *naver, int32_t *E, int32_t *D,
+                                               int32_t *M)
+{
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    int32_t e = 0, d = 0, m = 0;
+    for (uint32_t i = 0; i < N; i++) {
+        const int32_t s = lat[(uint64_t)i * W + w];
+        const uint32_t *nv = naver + 4u * i;
+        const int32_t nb = lat[(uint64_t)nv[0] * W + w] + lat[(uint64_t)nv[1] * W + w] +
+                           lat[(uint64_t)nv[2] * W + w] + lat[(uint64_t)nv[3] * W + w];
+        e += -s * nb;
+        d += s * s;
+        m += s;
+    }
+    E[w] = e >> 1;
+    D[w] = d;
+    if (M) M[w] = m;
+}
+
+hipError_t launch_state(hipStream_t st, const int8_t *lat, uint64_t W, uint32_t N,
+                        const uint32_t *naver, int32_t *E, int32_t *D, int32_t *M)
+{
+    hipLaunchKernelGGL(k_state, dim3((unsigned)((W + 255) / 256)), dim3(256), 0, st, lat, W, N,
+                       naver, E, D, M);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The update kernel, reference site order from tables (any L).
+// Per walker (muca_delta_multigpu.cu:735-829; produce2_multigpu.cu:401-496):
+//   keys  weight : k1 = {2w, 0xdecafbad}, k3 = {2w+50000, 0x11cafead}
+//         produce: k1 = {w, 0xdecafbad},  k3 = {w+40000, 0xdecafead}
+//   ctr = {++c0, seed, iter, 0xBADC0DED} once per four attempts, c0 shared
+//   by the therm and production phases, attempt index restarting at 0.
+//   step  = +1 if r3 < 2^31 (u01fixedpt<float>(r3) < 0.5) else -1, new in {-1,0,1}
+//   dE    = -(new - old) * (sum of the four neighbours)   [equals the
+//           reference's ((old==0) + old*(new-old)) * (-tmp) * sum for every
+//           old, new in {-1,0,1}, tmp being its temporary write]
+//   diff  = (T)(-dE*beta) + lnW[D+dDel] - lnW[D]   (T = float | double)
+//   accept iff diff >= 0 or acclog(r1) < diff
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T lnw_at(const McsArgs &a, int32_t i);
+template <>
+__device__ __forceinline__ float lnw_at<float>(const McsArgs &a, int32_t i)
+{
+    return a.lnw32[i];
+}
+template <>
+__device__ __forceinline__ double lnw_at<double>(const McsArgs &a, int32_t i)
+{
+    return a.lnw64[i];
+}
+
+template <int R, bool FP64, int KEYING, bool LDSHIST, bool JOINT>
+__global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
+{
+    typedef typename std::conditional<FP64, double, float>::type T;
+    extern __shared__ uint32_t lds_hist[];
+    const uint32_t nbins = a.N + 1;
+    if (LDSHIST) {
+        for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) lds_hist[b] = 0;
+        __syncthreads();
+    }
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < a.W) {
+        const uint32_t wid = a.w_offset + (uint32_t)w;
+        const uint32_t k1a = KEYING == 0 ? 2u * wid : wid;
+        const uint32_t k1b = 0xdecafbadu;
+        const uint32_t k3a = KEYING == 0 ? 2u * wid + 50000u : wid + 40000u;
+        const uint32_t k3b = KEYING == 0 ? 0x11cafeadu : 0xdecafeadu;
+        int8_t *s = a.lat + w;
+        const uint64_t W = a.W;
+        int32_t e = a.E[w], d = a.D[w], m = a.M[w];
+        uint64_t *jbase = nullptr;
+        if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
+        uint32_t c0 = 0, pos = 0;
+        for (int phase = 0; phase < 2; phase++) {
+            const uint64_t n = phase ? a.nup : a.therm;
+            for (uint64_t i0 = 0; i0 < n; i0 += 4) {
+                ++c0;
+                const philox4 r1 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k1a, k1b);
+                const philox4 r3 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k3a, k3b);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (i0 + j < n) {
+                        const uint32_t idx = a.order[pos];
+                        const uint32_t *nv = a.naver + 4u * idx;
+                        const int32_t old_s = s[(uint64_t)idx * W];
+                        int32_t new_s = old_s + (r3.v[j] < 0x80000000u ? 1 : -1);
+                        new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
+                        const int32_t nb = s[(uint64_t)nv[0] * W] + s[(uint64_t)nv[1] * W] +
+                                           s[(uint64_t)nv[2] * W] + s[(uint64_t)nv[3] * W];
+                        const int32_t dE = -(new_s - old_s) * nb;
+                        const int32_t dDel = new_s * new_s - old_s * old_s;
+                        // D stays in [0, N] for a consistent (lattice, D) pair; the
+                        // clamp keeps every table access in bounds for any input
+                        const int32_t dn = min(max(d + dDel, 0), (int32_t)a.N);
+                        T diff = (T)(-(double)dE * a.beta);
+                        diff = diff + lnw_at<T>(a, dn);
+                        diff = diff - lnw_at<T>(a, d);
+                        bool acc = diff >= (T)0;
+                        if (!acc) acc = (T)acclog_u01(r1.v[j]) < diff;
+                        if (acc) {
+                            s[(uint64_t)idx * W] = (int8_t)new_s;
+                            e += dE;
+                            d = dn;
+                            m += new_s - old_s;
+                        }
+                        if (phase) {
+                            if (LDSHIST)
+                                atomicAdd(&lds_hist[d], 1u);
+                            else
+                                atomicAdd((unsigned long long *)&a.hist[d], 1ull);
+                            if (JOINT) {
+                                const int64_t mm = min(max((int64_t)m, -(int64_t)d), (int64_t)d);
+                                const int64_t jb = (mm + (int64_t)d * d + 2 * (int64_t)d) / 2;
+                                atomicAdd((unsigned long long *)&jbase[jb], 1ull);
+                            }
+                        }
+                        pos = (pos + 1 == a.N) ? 0 : pos + 1;
+                    }
+                }
+            }
+            pos = 0;
+        }
+        a.E[w] = e;
+        a.D[w] = d;
+        a.M[w] = m;
+    }
+    if (LDSHIST) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) {
+            const uint32_t c = lds_hist[b];
+            if (c) atomicAdd((unsigned long long *)&a.hist[b], (unsigned long long)c);
+        }
+    }
+}
+
+template <int R, bool FP64, int KEYING>
+static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsArgs &a)
+{
+    const int block = cfg.block > 0 ? cfg.block : 256;
+    const dim3 grid((unsigned)((a.W + block - 1) / block));
+    const size_t lds = (size_t)(a.N + 1) * sizeof(uint32_t);
+    // uint32 LDS bins cannot overflow: a workgroup adds at most block * nup
+    const bool lds_ok = lds <= cfg.lds_max && (uint64_t)block * a.nup < (1ull << 32);
+    const bool joint = a.joint != nullptr;
+    if (lds_ok) {
+        if (joint) {
+            auto k = k_mcs_generic<R, FP64, KEYING, true, true>;
+            hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+            hipLaunchKernelGGL(k, grid, dim3(block), lds, st, a);
+        } else {
+            auto k = k_mcs_generic<R, FP64, KEYING, true, false>;
+            hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+            hipLaunchKernelGGL(k, grid, dim3(block), lds, st, a);
+        }
+    } else {
+        if (joint)
+            hipLaunchKernelGGL((k_mcs_generic<R, FP64, KEYING, false, true>), grid, dim3(block),
+                               0, st, a);
+        else
+            hipLaunchKernelGGL((k_mcs_generic<R, FP64, KEYING, false, false>), grid,
+                               dim3(block), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <bool FP64, int KEYING>
+static hipError_t launch_generic_k(hipStream_t st, const Launch &cfg, const McsArgs &a)
+{
+    return cfg.rounds == 10 ? launch_generic_r<10, FP64, KEYING>(st, cfg, a)
+                            : launch_generic_r<7, FP64, KEYING>(st, cfg, a);
+}
+
+hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a)
+{
+    if (a.W == 0 || (a.therm == 0 && a.nup == 0)) return hipSuccess;
+    if (cfg.fp64)
+        return cfg.keying == 0 ? launch_generic_k<true, 0>(st, cfg, a)
+                               : launch_generic_k<true, 1>(st, cfg, a);
+    return cfg.keying == 0 ? launch_generic_k<false, 0>(st, cfg, a)
+                           : launch_generic_k<false, 1>(st, cfg, a);
+}
+
+// ---------------------------------------------------------------------------
+// acceptance log evaluated on the device (self-check)
+// ---------------------------------------------------------------------------
+__global__ void k_acclog(uint32_t k0, uint32_t n, float *out)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = acclog_u01((k0 + t) << 9);
+}
+
+hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out)
+{
+    hipLaunchKernelGGL(k_acclog, dim3((n + 255) / 256), dim3(256), 0, st, k0, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// produce path: joint H(D,M) -> per-D moments, one wavefront per (block, D);
+// lanes stride over m so the reads of a shell are contiguous.
+// mom layout: [block][4][N+1] with S0 = sum H, S1 = sum |m| H, S2 = sum m^2 H,
+// S4 = sum m^4 H (FP64).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_joint_moments(const uint64_t *joint, uint32_t N,
+                                                       uint64_t stride, uint32_t blocks,
+                                                       double *mom)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t nb = N + 1;
+    if (wave >= nb * blocks) return;
+    const uint32_t b = wave / (uint32_t)nb;
+    const uint64_t D = wave % nb;
+    const uint64_t *h = joint + b * stride + (D * D + D) / 2;   // m = -D at offset 0
+    double a0 = 0, a1 = 0, a2 = 0, a4 = 0;
+    for (uint64_t j = lane; j <= D; j += 64) {
+        const double c = (double)h[j];
+        const double am = fabs((double)(2 * (int64_t)j - (int64_t)D));
+        const double m2 = am * am;
+        a0 += c;
+        a1 += am * c;
+        a2 += m2 * c;
+        a4 += m2 * m2 * c;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+        a2 += __shfl_down(a2, off, 64);
+        a4 += __shfl_down(a4, off, 64);
+    }
+    if (lane == 0) {
+        double *o = mom + (uint64_t)b * 4 * nb;
+        o[D] = a0;
+        o[nb + D] = a1;
+        o[2 * nb + D] = a2;
+        o[3 * nb + D] = a4;
+    }
+}
+
+hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t N,
+                                uint64_t stride, uint32_t blocks, double *mom)
+{
+    const uint64_t waves = (uint64_t)(N + 1) * blocks;
+    hipLaunchKernelGGL(k_joint_moments, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
+                       st, joint, N, stride, blocks, mom);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// reweighting (produce2_multigpu.cu:295-297,391-400,498-518), one workgroup
+// per (delta, block): shift = max_D(-lnW_D - beta*delta*D), then
+// Z = sum_D exp(-beta*delta*D - lnW_D - shift) S0_D, A_k likewise with S_k.
+// out[(id*blocks + b)*3 + 0..2] = <|M|>, chi, Binder cumulant.
+// ---------------------------------------------------------------------------
+__device__ static double block_reduce_sum(double v, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int wv = threadIdx.x / 64, ln = threadIdx.x & 63;
+    __syncthreads();
+    if (ln == 0) sh[wv] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x / 64); i++) t += sh[i];
+    return t;
+}
+
+__device__ static double block_reduce_max(double v, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    const int wv = threadIdx.x / 64, ln = threadIdx.x & 63;
+    __syncthreads();
+    if (ln == 0) sh[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int i = 1; i < (int)(blockDim.x / 64); i++) t = fmax(t, sh[i]);
+        sh[8] = t;
+    }
+    __syncthreads();
+    return sh[8];
+}
+
+__global__ __launch_bounds__(256) void k_reweight(const double *mom, const double *lnw,
+                                                  uint32_t N, uint32_t blocks, double beta,
+                                                  const double *deltas, double *out)
+{
+    __shared__ double sh[16];
+    const uint32_t id = blockIdx.x / blocks, b = blockIdx.x % blocks;
+    const double bd = beta * deltas[id];
+    const uint64_t nb = N + 1;
+    double mx = -INFINITY;
+    for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x)
+        mx = fmax(mx, -lnw[i] - bd * (double)i);
+    mx = block_reduce_max(mx, sh);
+    const double *S = mom + (uint64_t)b * 4 * nb;
+    double z = 0, a1 = 0, a2 = 0, a4 = 0;
+    for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+        const double bz = exp(-bd * (double)i - lnw[i] - mx);
+        z += bz * S[i];
+        a1 += bz * S[nb + i];
+        a2 += bz * S[2 * nb + i];
+        a4 += bz * S[3 * nb + i];
+    }
+    z = block_reduce_sum(z, sh);
+    a1 = block_reduce_sum(a1, sh);
+    a2 = block_reduce_sum(a2, sh);
+    a4 = block_reduce_sum(a4, sh);
+    if (threadIdx.x == 0) {
+        const double m1 = a1 / z, m2 = a2 / z, m4 = a4 / z;
+        double *o = out + ((uint64_t)id * blocks + b) * 3;
+        o[0] = m1;
+        o[1] = (m2 - m1 * m1) * beta / (double)N;
+        o[2] = 1.0 - m4 / (3.0 * m2 * m2);
+    }
+}
+
+hipError_t launch_reweight(hipStream_t st, const double *mom, const double *lnw, uint32_t N,
+                           uint32_t blocks, double beta, const double *deltas,
+                           uint32_t ndelta, double *out)
+{
+    hipLaunchKernelGGL(k_reweight, dim3(ndelta * blocks), dim3(256), 0, st, mom, lnw, N, blocks,
+                       beta, deltas, out);
+    return hipGetLastError();
+}
+
+}  // namespace muca

@@ -83,6 +83,11 @@ float orc_logf_u01(uint32_t r)
     return f;
 }
 
+void orc_logf_u01_range(uint32_t k0, uint32_t n, float *out)
+{
+    for (uint32_t t = 0; t < n; t++) out[t] = orc_logf_u01((k0 + t) << 9);
+}
+
 typedef struct {
     uint32_t k0, k1;
     long bad;

@@ -45,6 +45,8 @@ float orc_u01fixedpt_f32(uint32_t r);
 float orc_logf_u01(uint32_t r);
 /* exact reference: logq rounded to float (slow, ~1.5 us) */
 float orc_logf_u01_quad(uint32_t r);
+/* orc_logf_u01((k0+t) << 9) for t in [0, n) into out */
+void orc_logf_u01_range(uint32_t k0, uint32_t n, float *out);
 /* exhaustive check of orc_logf_u01 against orc_logf_u01_quad over all 2^23
  * distinct inputs, split over nthreads; returns the number of mismatches. */
 long orc_logf_u01_exhaustive(int nthreads);

@@ -55,6 +55,7 @@ def lib():
         L.orc_logf_u01.argtypes = [ctypes.c_uint32]
         L.orc_logf_u01_quad.restype = ctypes.c_float
         L.orc_logf_u01_quad.argtypes = [ctypes.c_uint32]
+        L.orc_logf_u01_range.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f32p]
         L.orc_logf_u01_exhaustive.restype = ctypes.c_long
         L.orc_logf_u01_exhaustive.argtypes = [ctypes.c_int]
         L.orc_tables.argtypes = [ctypes.c_uint32, _u32p, _u32p]
@@ -103,6 +104,12 @@ def logf_u01(r):
 
 def logf_u01_quad(r):
     return lib().orc_logf_u01_quad(r)
+
+
+def logf_u01_range(k0, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_logf_u01_range(k0, n, _p(out, _f32p))
+    return out
 
 
 def logf_u01_exhaustive(nthreads):

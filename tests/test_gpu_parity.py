@@ -1,0 +1,186 @@
+"""GPU parity: libmuca's HIP path against the CPU oracle, bit for bit.
+
+Lattices, E, D, M and every integer histogram must be identical for the same
+seed, walker ids, lattice and weights.  Oracle lattices are walker-major
+[walker][site]; the engine's are site-major [site][walker] (the reference's
+interleaving), hence the transposes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import muca_amd as M
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 4)
+
+
+def oracle_init(orc, keying, L, W, mm=0, gpu=0, seed=M.SEED, rounds=7):
+    lat = orc.init_lattice(keying, L, W, mm=mm, gpu=gpu, seed=seed, rounds=rounds)
+    E, D, Mg = orc.state(L, lat)
+    return lat, E, D, Mg
+
+
+def test_acclog_device_exhaustive(orc):
+    n, step = 1 << 23, 1 << 22
+    for k0 in range(0, n, step):
+        a = M.acclog_device(k0, step)
+        b = orc.logf_u01_range(k0, step)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k0
+
+
+@pytest.mark.parametrize("keying", [M.KEYING_WEIGHT, M.KEYING_PRODUCE])
+@pytest.mark.parametrize("L,W,mm,gpu,seed,rounds", [(16, 16, 0, 0, 1234, 7),
+                                                     (10, 300, 2, 1, 100, 7),
+                                                     (5, 7, 1, 0, 9, 10)])
+def test_init_lattice_and_state(orc, keying, L, W, mm, gpu, seed, rounds):
+    with M.Context(L, W, keying=keying, rounds=rounds) as ctx:
+        ctx.init_lattice(mm=mm, gpu=gpu, seed=seed)
+        ctx.compute_state()
+        lat = ctx.get_lattice()
+        E, D, Mg = ctx.get_state()
+    ref, rE, rD, rM = oracle_init(orc, keying, L, W, mm, gpu, seed, rounds)
+    assert np.array_equal(lat, ref.T)
+    assert np.array_equal(E, rE) and np.array_equal(D, rD) and np.array_equal(Mg, rM)
+
+
+CASES = [
+    # L, W, precision, keying, beta, rounds, therm, nup
+    (16, 16, M.REF32, M.KEYING_WEIGHT, 2.0, 7, 256, 1),
+    (16, 200, M.REF32, M.KEYING_WEIGHT, 2.0, 7, 256, 1027),
+    (12, 64, M.FP64, M.KEYING_PRODUCE, 2.0, 7, 144 * 3, 144 * 5 + 3),
+    (8, 33, M.REF32, M.KEYING_WEIGHT, 0.7, 10, 5, 299),
+    (8, 65, M.FP64, M.KEYING_WEIGHT, 1.3, 7, 0, 250),
+    (5, 20, M.REF32, M.KEYING_WEIGHT, 2.0, 7, 25, 61),       # odd L (reference order quirk)
+    (2, 9, M.FP64, M.KEYING_PRODUCE, 2.0, 7, 4, 33),
+    (24, 513, M.REF32, M.KEYING_PRODUCE, 2.0, 7, 576, 2),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "L%d_W%d_p%d_k%d_b%g_r%d_t%d_n%d" % c)
+def test_single_launch_parity(orc, case):
+    L, W, prec, keying, beta, rounds, therm, nup = case
+    N = L * L
+    rng = np.random.default_rng(L * 1000 + W)
+    lnw = np.cumsum(rng.normal(scale=1.5, size=N + 1))
+    lnw -= lnw[0]
+    lat, E, D, Mg = oracle_init(orc, keying, L, W, seed=77, rounds=rounds)
+    with M.Context(L, W, keying=keying, precision=prec, beta=beta, rounds=rounds) as ctx:
+        ctx.set_lattice(lat.T)
+        ctx.set_state(E, D, Mg)
+        ctx.set_log_weight(lnw)
+        hist, _ = ctx.run(3, 1234, therm, nup)
+        glat = ctx.get_lattice()
+        gE, gD, _ = ctx.get_state()
+    h = orc.mcs(L, lat, E, D, Mg, keying=keying, fp64=prec == M.FP64,
+                lnw32=lnw.astype(np.float32), lnw64=lnw, iter=3, seed=1234, therm=therm,
+                nupdates=nup, beta=beta, rounds=rounds, nthreads=NTHREADS)
+    assert np.array_equal(glat, lat.T)
+    assert np.array_equal(gE, E) and np.array_equal(gD, D)
+    assert np.array_equal(hist, h)
+    assert int(hist.sum()) == W * nup
+
+
+def test_zero_attempts_leaves_state(orc):
+    L, W = 8, 40
+    with M.Context(L, W) as ctx:
+        ctx.init_lattice(seed=5)
+        ctx.compute_state()
+        ctx.set_log_weight(np.zeros(L * L + 1))
+        before = ctx.get_lattice()
+        hist, _ = ctx.run(0, 100, 0, 0)
+        assert np.array_equal(before, ctx.get_lattice())
+        assert int(hist.sum()) == 0
+
+
+def _weight_loop_parity(orc, L, W, iters, seed, recursion=False):
+    recs = orc.weight_iteration_reference(L, W, iters, seed=seed, recursion=recursion,
+                                          nthreads=NTHREADS)
+    with M.Context(L, W) as ctx:
+        ctx.init_lattice(mm=0, gpu=0, seed=seed)
+        ctx.compute_state()
+        got = []
+        M.weight_iteration(ctx, iters, seed=seed, recursion=recursion,
+                           callback=lambda r: got.append(
+                               dict(r, lat=ctx.get_lattice(), ED=ctx.get_state())))
+    assert len(got) == len(recs)
+    for g, r in zip(got, recs):
+        assert np.array_equal(g["hist"], r["hist"]), g["k"]
+        assert np.array_equal(g["lat"], r["lat"].T), g["k"]
+        assert np.array_equal(g["ED"][0], r["E"]) and np.array_equal(g["ED"][1], r["D"])
+        assert g["width"] == r["width"] and g["nup"] == r["nup"] and g["dkl"] == r["dkl"]
+        assert np.array_equal(g["lnw"], r["lnw"]), g["k"]
+
+
+def test_weight_loop_L16_16walkers_10iters(orc):
+    """Plain method, small: L=16, 16 walkers, seed 1234, 10 trivial iterations."""
+    _weight_loop_parity(orc, 16, 16, 10, 1234)
+
+
+def test_weight_loop_recursive_L16(orc):
+    _weight_loop_parity(orc, 16, 64, 12, 1234, recursion=True)
+
+
+def test_weight_loop_L72_4096walkers_20iters(orc):
+    """Plain method at reference scale: L=72, 4096 walkers, 20 iterations."""
+    _weight_loop_parity(orc, 72, 4096, 20, 1234)
+
+
+def test_produce_joint_histograms_per_block(orc):
+    L, W, B = 8, 96, 3
+    N = L * L
+    rng = np.random.default_rng(5)
+    lnw = np.cumsum(rng.normal(size=N + 1))
+    lat, E, D, Mg = oracle_init(orc, M.KEYING_PRODUCE, L, W, seed=100)
+    with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B) as ctx:
+        ctx.set_lattice(lat.T)
+        ctx.set_state(E, D, Mg)
+        ctx.set_log_weight(lnw)
+        ctx.run(0, 101, 2 * N, 0)          # therm-only launch: no joint accumulation
+        ctx.run(1, 200, 0, 7 * N + 1)
+        joints = [ctx.get_joint(b) for b in range(B)]
+        _, _, gM = ctx.get_state()
+    kw = dict(keying=M.KEYING_PRODUCE, fp64=True, lnw64=lnw, nthreads=1)
+    orc.mcs(L, lat, E, D, Mg, iter=0, seed=101, therm=2 * N, nupdates=0, **kw)
+    bw = W // B
+    for b in range(B):
+        j = np.zeros((N + 1) * (N + 2) // 2, np.uint64)
+        sub = np.ascontiguousarray(lat[b * bw:(b + 1) * bw])
+        e, d, m = E[b * bw:(b + 1) * bw].copy(), D[b * bw:(b + 1) * bw].copy(), \
+            Mg[b * bw:(b + 1) * bw].copy()
+        orc.mcs(L, sub, e, d, m, iter=1, seed=200, therm=0, nupdates=7 * N + 1, w0=b * bw,
+                joint=j, **kw)
+        assert np.array_equal(joints[b], j), b
+        assert np.array_equal(gM[b * bw:(b + 1) * bw], m)
+
+
+def test_reweighting_matches_oracle_observables(orc):
+    """Device FP64 reweighting of the joint histograms vs the reference's
+    long-double observables() (produce2_multigpu.cu:498-518)."""
+    L, W, B = 8, 128, 4
+    N = L * L
+    lnw = -np.log(np.arange(1, N + 2, dtype=np.float64)) * 3.0
+    lnw -= lnw[0]
+    deltas = np.array([1.9, 1.95, 1.98, 2.0])
+    with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B) as ctx:
+        ctx.init_lattice(seed=100)
+        ctx.compute_state()
+        ctx.set_log_weight(lnw)
+        ctx.run(0, 100, 50 * N, 0)
+        ctx.run(1, 2100, 0, 100 * N)
+        absM, chi, binder = ctx.reweight(deltas)
+        joints = [ctx.get_joint(b) for b in range(B)]
+    for i, dl in enumerate(deltas):
+        w = lnw.copy()
+        orc.weight_shift(2.0, dl, w)
+        bz = orc.boltz(2.0, dl, w)
+        for b in range(B):
+            a_ref, c_ref = orc.observables(2.0, N, joints[b], bz)
+            assert absM[i, b] == pytest.approx(a_ref, rel=1e-12)
+            assert chi[i, b] == pytest.approx(c_ref, rel=1e-9)
+            S0, S1, S2, S4 = orc.joint_moments(N, joints[b])
+            z = (bz * S0).sum()
+            m2, m4 = (bz * S2).sum() / z, (bz * S4).sum() / z
+            assert binder[i, b] == pytest.approx(1 - m4 / (3 * m2 * m2), rel=1e-10)

@@ -1,0 +1,86 @@
+"""libmuca without a GPU: the library loads, exports every symbol that
+include/muca.h declares, and its host-side code agrees with the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import muca_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "muca.h")).read()
+    return sorted(set(re.findall(r"\b(muca_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol():
+    lib = M.lib()
+    syms = header_symbols()
+    assert len(syms) >= 24
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(M._SIGS), set(syms) ^ set(M._SIGS)
+
+
+def test_acclog_host_instantiation_exhaustive(orc):
+    """The acceptance log compiled for the host equals the oracle's correctly
+    rounded log on every one of the 2^23 inputs."""
+    n = 1 << 23
+    step = 1 << 21
+    for k0 in range(0, n, step):
+        a = M.acclog_host_range(k0, step)
+        b = orc.logf_u01_range(k0, step)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k0
+
+
+def _random_hists(rng, N, count=20):
+    for t in range(count):
+        h = rng.integers(0, 50, size=N + 1).astype(np.uint64)
+        if t % 3 == 0:
+            h[: rng.integers(0, N)] = 0
+        if t % 4 == 1:
+            h[rng.integers(0, N):] = rng.integers(0, 2)
+        if t % 5 == 2:
+            h[:] = 1
+        yield h
+
+
+def test_host_weight_pieces_match_oracle(orc):
+    rng = np.random.default_rng(42)
+    N = 64
+    for h in _random_hists(rng, N):
+        assert M.hist_range(h) == orc.hist_range(h)
+        a, b = M.kull_leibler(h), orc.kull_leibler(h)
+        assert (np.isnan(a) and np.isnan(b)) or a == b
+        lnw0 = rng.normal(size=N + 1) * 10
+        x, y = lnw0.copy(), lnw0.copy()
+        M.update_trivial(h, x)
+        orc.update_trivial(h, y)
+        assert np.array_equal(x, y)
+        s, e = M.hist_range(h)
+        pa, pb = rng.random(N + 1), None
+        pb = pa.copy()
+        x, y = lnw0.copy(), lnw0.copy()
+        M.update_recursive(h, s, e, pa, x)
+        orc.weight_recursion(h, pb, s, e, y)
+        assert np.array_equal(x, y) and np.array_equal(pa, pb)
+        M.normalise(x)
+        orc.normalise(y)
+        assert np.array_equal(x, y)
+
+
+def test_jackknife_matches_oracle(orc):
+    rng = np.random.default_rng(0)
+    for J in (2, 3, 10):
+        x = rng.normal(size=J)
+        assert M.jackknife(x) == orc.jackknife(x)
+
+
+def test_bad_arguments_raise():
+    with pytest.raises(M.MucaError):
+        M.update_recursive(np.ones(5, np.uint64), 3, 1, np.zeros(5), np.zeros(5))
+    with pytest.raises(M.MucaError):
+        M.jackknife(np.zeros(0))
