@@ -102,6 +102,9 @@ int muca_set_log_weight(muca_ctx *ctx, const double *lnw, uint32_t nbins);
  * accumulated too (read it with muca_get_joint). */
 int muca_run(muca_ctx *ctx, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t nupdates,
              uint64_t *hist_out, float *kernel_ms);
+/* D2H of the last launch's D histogram (muca_delta_multigpu.cu:401-412);
+ * copy_ms (nullable) receives the copy's duration */
+int muca_get_hist(muca_ctx *ctx, uint64_t *hist, uint32_t nbins, float *copy_ms);
 /* joint histogram of one walker block: (N+1)(N+2)/2 bins at (M+D*D+2*D)/2 */
 int muca_get_joint(muca_ctx *ctx, uint32_t block, uint64_t *joint, uint64_t nbins);
 

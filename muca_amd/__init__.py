@@ -68,6 +68,7 @@ _SIGS = {
     "muca_set_log_weight": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_uint32]),
     "muca_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint64, ctypes.c_uint64, _u64p, _f32p]),
+    "muca_get_hist": (ctypes.c_int, [ctypes.c_void_p, _u64p, ctypes.c_uint32, _f32p]),
     "muca_get_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
                                       ctypes.c_uint64]),
     "muca_hist_range": (ctypes.c_int, [_u64p, ctypes.c_uint32, _i32p, _i32p]),

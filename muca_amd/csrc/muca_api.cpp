@@ -355,6 +355,19 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     return MUCA_OK;
 }
 
+int muca_get_hist(muca_ctx *c, uint64_t *hist, uint32_t nbins, float *copy_ms)
+{
+    if (!c || !hist) return fail(MUCA_E_ARG, "NULL argument");
+    if (nbins != c->N + 1) return fail(MUCA_E_ARG, "nbins must be N+1");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipEventRecord(c->ev0, c->st));
+    HIP_TRY(hipMemcpyAsync(hist, c->hist, (uint64_t)nbins * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipEventRecord(c->ev1, c->st));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    if (copy_ms) HIP_TRY(hipEventElapsedTime(copy_ms, c->ev0, c->ev1));
+    return MUCA_OK;
+}
+
 int muca_get_joint(muca_ctx *c, uint32_t block, uint64_t *dst, uint64_t nbins)
 {
     if (!c || !dst) return fail(MUCA_E_ARG, "NULL argument");
