@@ -42,7 +42,7 @@ extern "C" {
 /* update-kernel variants (MUCA_KERNEL_AUTO lets the autotuner choose) */
 #define MUCA_KERNEL_AUTO 0
 #define MUCA_KERNEL_GENERIC 1  /* one lane per walker, reference site order from tables */
-#define MUCA_KERNEL_STRIPE 2   /* one lane per walker, even-L stripe order, register weight window */
+#define MUCA_KERNEL_TEAM 2     /* sixteen lanes per walker, LDS-resident lattices (L % 8 == 0) */
 
 typedef struct muca_ctx muca_ctx;
 
@@ -64,6 +64,8 @@ typedef struct muca_info {
     int32_t device, kernel;
     uint32_t cus, block;          /* compute units, threads per block of the update */
     uint64_t lattice_bytes, joint_bins;
+    uint32_t lds_per_block;       /* bytes of LDS one workgroup may use */
+    uint32_t lds_lattice;         /* 1 if the team kernel keeps lattices in LDS */
 } muca_info;
 
 const char *muca_last_error(void);

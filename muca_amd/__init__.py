@@ -21,7 +21,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 
 REF32, FP64 = 0, 1
 KEYING_WEIGHT, KEYING_PRODUCE = 0, 1
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_STRIPE = 0, 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_TEAM = 0, 1, 2
 SEED = 100                       # muca_delta_multigpu.cu:19
 
 _u8p = ctypes.POINTER(ctypes.c_int8)
@@ -48,7 +48,8 @@ class Info(ctypes.Structure):
                 ("nbins", ctypes.c_uint32), ("device", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("cus", ctypes.c_uint32),
                 ("block", ctypes.c_uint32), ("lattice_bytes", ctypes.c_uint64),
-                ("joint_bins", ctypes.c_uint64)]
+                ("joint_bins", ctypes.c_uint64), ("lds_per_block", ctypes.c_uint32),
+                ("lds_lattice", ctypes.c_uint32)]
 
 
 # every symbol declared in include/muca.h, with its ctypes signature

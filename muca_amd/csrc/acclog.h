@@ -36,8 +36,12 @@ __host__ __device__ static inline uint64_t acclog_double_to_bits(double d)
     return b;
 }
 
-// log(u) in double-double precision, u = m * 2^-24 with m odd in [1, 2^24).
-__host__ __device__ static inline double acclog_u01_double(uint32_t r)
+// log(u) in double-double precision, u = m * 2^-24 with m odd in [1, 2^24),
+// with the table at (invc_tab, logc_tab) (device constant memory, host
+// memory, or a copy in LDS).
+__host__ __device__ static inline double acclog_u01_double_tab(uint32_t r,
+                                                              const double *invc_tab,
+                                                              const double *logc_tab)
 {
     const uint32_t m = 1u | (r >> 8);
     const double x = (double)m * 0x1p-24;
@@ -46,13 +50,8 @@ __host__ __device__ static inline double acclog_u01_double(uint32_t r)
     const int i = (int)((tmp >> 45) & 127u);
     const int64_t k = (int64_t)tmp >> 52;
     const double z = acclog_bits_to_double(ix - (tmp & (0xfffULL << 52)));
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double invc = acclog_invc_dev[i];
-    const double logc = acclog_logc_dev[i];
-#else
-    const double invc = acclog_invc_host[i];
-    const double logc = acclog_logc_host[i];
-#endif
+    const double invc = invc_tab[i];
+    const double logc = logc_tab[i];
     const double rr = fma(z, invc, -1.0);           // |rr| <= 2^-7
     const double kf = (double)k;
     const double kh = kf * ACCLOG_LN2HI;             // exact: |k| <= 24
@@ -74,7 +73,25 @@ __host__ __device__ static inline double acclog_u01_double(uint32_t r)
     return t2 + lo;
 }
 
+__host__ __device__ static inline double acclog_u01_double(uint32_t r)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return acclog_u01_double_tab(r, acclog_invc_dev, acclog_logc_dev);
+#else
+    return acclog_u01_double_tab(r, acclog_invc_host, acclog_logc_host);
+#endif
+}
+
 __host__ __device__ static inline float acclog_u01(uint32_t r)
 {
     return (float)acclog_u01_double(r);
+}
+
+// device: copy the 128-entry table into LDS (invc at dst[0..127], logc at dst[128..255])
+__device__ static inline void acclog_table_to_lds(double *dst, int tid, int nthreads)
+{
+    for (int i = tid; i < 128; i += nthreads) {
+        dst[i] = acclog_invc_dev[i];
+        dst[128 + i] = acclog_logc_dev[i];
+    }
 }

@@ -43,6 +43,21 @@ hipError_t launch_init_lattice(hipStream_t st, int8_t *lat, uint64_t W, uint32_t
 hipError_t launch_state(hipStream_t st, const int8_t *lat, uint64_t W, uint32_t N,
                         const uint32_t *naver, int32_t *E, int32_t *D, int32_t *M);
 hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a);
+
+// team kernel (muca_team.hip): LDS layout of one workgroup of 16 walkers
+struct TeamLayout {
+    uint32_t row;         // L/2 attempts per half-row
+    uint32_t team_bytes;  // per-team Philox words, attempt records and D sequence
+    uint32_t off_rec, off_dseq;  // offsets inside a team's area
+    uint32_t off_team, off_tab, off_w, off_hist, off_lat;
+    size_t bytes;         // dynamic LDS per workgroup
+    bool lds_hist, lat_lds;
+};
+// false when the team kernel cannot run this launch (L % 8 != 0, L > 256,
+// or ln W does not fit in LDS)
+bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay);
+hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
+                           const TeamLayout &lay);
 hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 
 // joint H(D,M) -> per-D moments (S0, S1, S2, S4) for each block, FP64

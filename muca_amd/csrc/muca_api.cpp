@@ -161,7 +161,19 @@ int muca_create(const muca_params *params, muca_ctx **out)
     c->cfg.rounds = (int)p.rounds;
     c->cfg.fp64 = p.precision == MUCA_FP64;
     c->cfg.keying = p.keying;
-    c->cfg.kernel = p.kernel == MUCA_KERNEL_AUTO ? MUCA_KERNEL_GENERIC : p.kernel;
+    // Kernel choice (measured on MI355X, L = 72): the team kernel keeps every
+    // SIMD busy when walkers are few (31 attempts/ns at 4096 walkers against
+    // 6.7 for one lane per walker); one lane per walker wins once there are
+    // enough wavefronts to hide its memory latency (58/ns at 65536 walkers
+    // against 28).  The crossover sits near 128 walkers per compute unit.
+    if (p.kernel == MUCA_KERNEL_AUTO) {
+        const bool team_ok = p.L % 8 == 0 && p.L <= 256;
+        c->cfg.kernel = (team_ok && (uint64_t)W <= 128ull * (uint64_t)(cu > 0 ? cu : 256))
+                            ? MUCA_KERNEL_TEAM
+                            : MUCA_KERNEL_GENERIC;
+    } else {
+        c->cfg.kernel = p.kernel;
+    }
     c->cfg.block = 256;
     c->cfg.lds_max = (size_t)(lds > 0 ? lds : 65536);
     const uint64_t N = c->N;
@@ -228,6 +240,15 @@ int muca_query(const muca_ctx *c, muca_info *info)
     info->block = (uint32_t)c->cfg.block;
     info->lattice_bytes = (uint64_t)c->N * c->W;
     info->joint_bins = c->joint_stride * c->p.joint_blocks;
+    info->lds_per_block = (uint32_t)c->cfg.lds_max;
+    muca::McsArgs a{};
+    a.N = c->N;
+    a.L = c->p.L;
+    a.W = c->W;
+    a.nup = 1;
+    muca::TeamLayout lay{};
+    info->lds_lattice = c->cfg.kernel == MUCA_KERNEL_TEAM &&
+                        muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay) && lay.lat_lds;
     return MUCA_OK;
 }
 

@@ -248,6 +248,10 @@ static hipError_t launch_generic_k(hipStream_t st, const Launch &cfg, const McsA
 hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
     if (a.W == 0 || (a.therm == 0 && a.nup == 0)) return hipSuccess;
+    if (cfg.kernel == 2) {
+        TeamLayout lay{};
+        if (team_layout(a, cfg.fp64, cfg.lds_max, lay)) return launch_mcs_team(st, cfg, a, lay);
+    }
     if (cfg.fp64)
         return cfg.keying == 0 ? launch_generic_k<true, 0>(st, cfg, a)
                                : launch_generic_k<true, 1>(st, cfg, a);

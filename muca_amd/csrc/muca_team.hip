@@ -1,0 +1,386 @@
+// muca_team.hip -- the team update kernel: sixteen lanes (one DPP row) per
+// walker, four walkers per wavefront, sixteen per workgroup.
+//
+// Why: a walker is a strict serial chain, so with few walkers (4096 walkers
+// fill only 64 wavefronts) one-lane-per-walker leaves most SIMDs idle and
+// every attempt waits on a chain of dependent memory loads.  The reference's
+// even-L site order is column-parity stripes (muca_delta_multigpu.cu:173):
+// a half-row of L/2 same-parity sites contains no two neighbours, and every
+// neighbour of a half-row was last written by an earlier half-row.  So per
+// half-row the team
+//   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
+//      the proposed spin, dE, dDel, (T)(-dE*beta), the acceptance log and an
+//      acceptance bitmask over a 16-wide window of D values around the D the
+//      row starts from;
+//   B. runs only the D chain serially: attempt j's mask is broadcast from
+//      its lane with a DPP row broadcast and the accept bit is read at the
+//      current D -- no memory access on the chain.  An attempt whose D has
+//      left the window (more than 8 net moves in one half-row) takes an
+//      exact slow path from LDS;
+//   C. writes the accepted spins, sums dE and dM and fills the histogram.
+// Every acceptance is computed with the reference's operations and rounding:
+//   diff = ((T)(-dE*beta) + lnW[D+dDel]) - lnW[D];  accept iff diff >= 0 or
+//   acclog(r1) < diff.  Because acclog(r1) < 0 always, that is exactly
+//   acclog(r1) < diff.  Results are bit-identical to the one-lane kernel.
+// Requires L % 8 == 0 (whole Philox blocks per half-row) and L <= 256.
+//
+// LDS (dynamic): per team the Philox words [2][L/2] and attempt records
+// [L/2] (slow path); the acceptance-log table; ln W with guard entries on
+// each side; the D histogram (u32, when it fits); the workgroup's lattices
+// [N][16] int8 (when they fit).
+#include "kernels.h"
+
+#include "acclog.h"
+#include "philox.h"
+
+namespace muca {
+
+namespace {
+
+constexpr int kTeam = 16;
+constexpr int kTeams = 16;   // per workgroup of 256 lanes
+constexpr int kWin = 16;     // D values per acceptance mask
+constexpr int kGuard = 24;   // ln W guard entries on each side (> kWin / 2 + 2)
+
+template <typename T>
+struct TeamRec {
+    T fde;          // (T)(-dE * beta)
+    float logu;     // acceptance log of r1
+    int32_t ddel;   // new^2 - old^2
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int32_t team_sum(int32_t v)
+{
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, kTeam);
+    return v;
+}
+
+// value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L)
+template <int L>
+__device__ __forceinline__ int32_t row_bcast(int32_t v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, 0x150 + L, 0xf, 0xf, false);
+}
+
+}  // namespace
+
+// NR: rounds of 16 attempts per half-row, NR * 16 >= L / 2
+template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
+__global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
+{
+    typedef typename std::conditional<FP64, double, float>::type T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    T *wt = (T *)(lds + lay.off_w) + kGuard;   // valid for [-kGuard, N + kGuard]
+    uint32_t *hist = (uint32_t *)(lds + lay.off_hist);
+    int8_t *latl = (int8_t *)(lds + lay.off_lat);
+    double *logtab = (double *)(lds + lay.off_tab);
+    const int tid = threadIdx.x, team = tid >> 4, tl = tid & 15;
+    const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
+    unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
+    uint32_t *rng = (uint32_t *)tbase;                          // [2][Rw]
+    TeamRec<T> *rec = (TeamRec<T> *)(tbase + lay.off_rec);      // [Rw]
+
+    const uint64_t W = a.W;
+    const uint64_t wbase = (uint64_t)blockIdx.x * kTeams;
+    const uint64_t w = wbase + team;
+    const bool active = w < W;
+
+    for (int i = (int)tid - kGuard; i <= (int)N + kGuard; i += blockDim.x) {
+        T v = (T)0;
+        if (i >= 0 && i <= (int)N) v = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
+        wt[i] = v;
+    }
+    if (LDSHIST)
+        for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = 0;
+    acclog_table_to_lds(logtab, tid, blockDim.x);
+    if (LATLDS) {
+        for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
+            const uint32_t site = e >> 4, t = e & 15;
+            latl[e] = (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
+        }
+    }
+    __syncthreads();
+
+    if (active) {
+        auto sp = [&](uint32_t site) -> int8_t & {
+            return LATLDS ? latl[site * kTeams + team] : a.lat[(uint64_t)site * W + w];
+        };
+        const uint32_t wid = a.w_offset + (uint32_t)w;
+        const uint32_t k1a = KEYING == 0 ? 2u * wid : wid;
+        const uint32_t k1b = 0xdecafbadu;
+        const uint32_t k3a = KEYING == 0 ? 2u * wid + 50000u : wid + 40000u;
+        const uint32_t k3b = KEYING == 0 ? 0x11cafeadu : 0xdecafeadu;
+        int32_t d = a.D[w], e = a.E[w], m = a.M[w];
+        uint64_t *jbase = nullptr;
+        if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
+        uint32_t c0base = 0;
+        for (int phase = 0; phase < 2; phase++) {
+            const uint64_t n = phase ? a.nup : a.therm;
+            for (uint64_t rs = 0; rs < n; rs += Rw) {
+                const uint32_t cnt = (uint32_t)min((uint64_t)Rw, n - rs);
+                const uint32_t pos0 = (uint32_t)(rs % N);
+                const uint32_t h = pos0 / NH;
+                const uint32_t y = (pos0 - h * NH) / Rw;
+                // ---- A1: Philox blocks of this half-row, one (block, key) per lane
+                const uint32_t nblk = (cnt + 3) / 4;
+                const uint32_t blk0 = c0base + (uint32_t)(rs / 4);
+                for (uint32_t task = tl; task < 2 * nblk; task += kTeam) {
+                    const uint32_t b = task >> 1, key = task & 1;
+                    const philox4 r = philox4x32<R>(blk0 + b + 1, a.seed, a.iter, 0xBADC0DEDu,
+                                                    key ? k3a : k1a, key ? k3b : k1b);
+                    uint32_t *dst = rng + key * Rw + 4 * b;
+                    dst[0] = r.v[0];
+                    dst[1] = r.v[1];
+                    dst[2] = r.v[2];
+                    dst[3] = r.v[3];
+                }
+                wave_sync();
+                // ---- A2: attempt j = 16 r + tl on this lane, everything in registers
+                const int32_t dlo = d - kWin / 2;     // mask bit k <-> D = dlo + k
+                T win[kWin + 2];                      // lnW[dlo - 1 .. dlo + kWin]
+#pragma unroll
+                for (int k = 0; k < kWin + 2; k++) win[k] = wt[dlo - 1 + k];
+                const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
+                uint32_t mask[NR];
+                int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const uint32_t j = 16 * r + tl;
+                    mask[r] = 0;
+                    info[r] = (8 << 2) | (2 << 7);
+                    if (j < cnt) {
+                        const uint32_t x = 2 * j + h;
+                        const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
+                        const uint32_t xl = (x == 0) ? L - 1 : x - 1;
+                        const int32_t old_s = sp(y * L + x);
+                        const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
+                                           sp(yd * L + x);
+                        const uint32_t r1 = rng[j], r3 = rng[Rw + j];
+                        int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
+                        new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
+                        const int32_t ds = new_s - old_s;
+                        const int32_t dE = -ds * nb;
+                        const int32_t ddel = new_s * new_s - old_s * old_s;
+                        const T fde = (T)(-(double)dE * a.beta);
+                        const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
+                        const T tlog = (T)logu;
+                        uint32_t mk = 0;
+#pragma unroll
+                        for (int k = kWin - 1; k >= 0; k--) {
+                            const T wa = ddel > 0 ? win[k + 2] : (ddel < 0 ? win[k] : win[k + 1]);
+                            T diff = fde + wa;
+                            diff = diff - win[k + 1];
+                            mk = mk + mk + (tlog < diff ? 1u : 0u);
+                        }
+                        mask[r] = mk;
+                        info[r] = (ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7);
+                        TeamRec<T> rc;
+                        rc.fde = fde;
+                        rc.logu = logu;
+                        rc.ddel = ddel;
+                        rec[j] = rc;
+                    }
+                }
+                wave_sync();
+                // ---- B: the D chain
+                uint32_t accb[NR];
+                int32_t dv[NR], mv[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    accb[r] = 0;
+                    dv[r] = 0;
+                    mv[r] = 0;
+                }
+                auto step = [&](auto lc, int r) {
+                    constexpr int l = decltype(lc)::value;
+                    const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
+                    const int32_t ij = row_bcast<l>(info[r]);
+                    const int32_t ddel = __builtin_amdgcn_sbfe(ij, 0, 2);
+                    const uint32_t off = (uint32_t)(d - dlo);
+                    uint32_t acc = (mj >> (off & 31u)) & 1u;
+                    const bool oob = off >= (uint32_t)kWin;
+                    if (__builtin_amdgcn_ballot_w64(oob)) {   // rare: D left the window
+                        const TeamRec<T> rc = rec[16 * r + l];
+                        T diff = rc.fde + wt[d + rc.ddel];
+                        diff = diff - wt[d];
+                        const uint32_t slow = (T)rc.logu < diff ? 1u : 0u;
+                        acc = oob ? slow : acc;
+                    }
+                    d += (int32_t)acc * ddel;
+                    accb[r] |= acc << l;
+                    dv[r] = tl == l ? d : dv[r];
+                    if (JOINT) {
+                        m += (int32_t)acc * (((ij >> 7) & 7) - 2);
+                        mv[r] = tl == l ? m : mv[r];
+                    }
+                };
+#define MUCA_STEP(LL) step(std::integral_constant<int, (LL)>(), r);
+#define MUCA_STEP_IF(LL) \
+    if ((LL) < left) step(std::integral_constant<int, (LL)>(), r);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const uint32_t j0 = 16 * r;
+                    if (j0 >= cnt) break;
+                    const uint32_t left = cnt - j0;
+                    if (left >= 16) {
+                        MUCA_STEP(0) MUCA_STEP(1) MUCA_STEP(2) MUCA_STEP(3)
+                        MUCA_STEP(4) MUCA_STEP(5) MUCA_STEP(6) MUCA_STEP(7)
+                        MUCA_STEP(8) MUCA_STEP(9) MUCA_STEP(10) MUCA_STEP(11)
+                        MUCA_STEP(12) MUCA_STEP(13) MUCA_STEP(14) MUCA_STEP(15)
+                    } else {
+                        MUCA_STEP_IF(0) MUCA_STEP_IF(1) MUCA_STEP_IF(2) MUCA_STEP_IF(3)
+                        MUCA_STEP_IF(4) MUCA_STEP_IF(5) MUCA_STEP_IF(6) MUCA_STEP_IF(7)
+                        MUCA_STEP_IF(8) MUCA_STEP_IF(9) MUCA_STEP_IF(10) MUCA_STEP_IF(11)
+                        MUCA_STEP_IF(12) MUCA_STEP_IF(13) MUCA_STEP_IF(14)
+                    }
+                }
+#undef MUCA_STEP_IF
+#undef MUCA_STEP
+                // ---- C: accepted spins, dE / dM sums, histogram
+                int32_t de = 0, dm = 0;
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const uint32_t j = 16 * r + tl;
+                    if (j < cnt) {
+                        if ((accb[r] >> tl) & 1u) {
+                            const int32_t ds = ((info[r] >> 7) & 7) - 2;
+                            int8_t &s = sp(y * L + 2 * j + h);
+                            s = (int8_t)(s + ds);
+                            de += ((info[r] >> 2) & 31) - 8;
+                            dm += ds;
+                        }
+                        if (phase) {
+                            if (LDSHIST)
+                                atomicAdd(&hist[dv[r]], 1u);
+                            else
+                                atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull);
+                            if (JOINT) {
+                                const int64_t dd = dv[r];
+                                const int64_t mm = min(max((int64_t)mv[r], -dd), dd);
+                                atomicAdd((unsigned long long *)&jbase[(mm + dd * dd + 2 * dd) / 2],
+                                          1ull);
+                            }
+                        }
+                    }
+                }
+                e += team_sum(de);
+                const int32_t dmt = team_sum(dm);
+                if (!JOINT) m += dmt;
+                if (!LATLDS) __threadfence_block();
+                wave_sync();
+            }
+            c0base += (uint32_t)((n + 3) / 4);
+        }
+        if (tl == 0) {
+            a.D[w] = d;
+            a.E[w] = e;
+            a.M[w] = m;
+        }
+    }
+    __syncthreads();
+    if (LDSHIST) {
+        for (uint32_t i = tid; i <= N; i += blockDim.x) {
+            const uint32_t c = hist[i];
+            if (c) atomicAdd((unsigned long long *)&a.hist[i], (unsigned long long)c);
+        }
+    }
+    if (LATLDS) {
+        for (uint32_t e2 = tid; e2 < N * kTeams; e2 += blockDim.x) {
+            const uint32_t site = e2 >> 4, t = e2 & 15;
+            if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[e2];
+        }
+    }
+}
+
+bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
+{
+    if (a.L % 8 != 0 || a.L > 256) return false;
+    const size_t tsz = fp64 ? 8 : 4;
+    const size_t row = a.L / 2;
+    const size_t recsz = fp64 ? sizeof(TeamRec<double>) : sizeof(TeamRec<float>);
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    lay.row = (uint32_t)row;
+    lay.off_rec = (uint32_t)al(2 * row * 4);
+    lay.off_dseq = 0;
+    lay.team_bytes = (uint32_t)(lay.off_rec + al(row * recsz));
+    size_t off = 0;
+    lay.off_team = 0;
+    off += al((size_t)lay.team_bytes * kTeams);
+    lay.off_tab = (uint32_t)off;
+    off += 256 * sizeof(double);
+    lay.off_w = (uint32_t)off;
+    off += al((a.N + 1 + 2 * kGuard) * tsz);
+    // histogram and lattice in LDS when they fit (histogram first)
+    const bool hist_fits = off + al((a.N + 1) * 4) <= lds_max &&
+                           (uint64_t)kTeams * a.nup < (1ull << 32);
+    lay.lds_hist = hist_fits;
+    lay.off_hist = (uint32_t)off;
+    if (hist_fits) off += al((a.N + 1) * 4);
+    lay.off_lat = (uint32_t)off;
+    lay.lat_lds = off + al((size_t)a.N * kTeams) <= lds_max;
+    if (lay.lat_lds) off += al((size_t)a.N * kTeams);
+    lay.bytes = off;
+    return off <= lds_max;
+}
+
+template <int R, bool FP64, int KEYING, int NR>
+static hipError_t launch_team_nr(hipStream_t st, const McsArgs &a, const TeamLayout &lay)
+{
+    const dim3 grid((unsigned)((a.W + kTeams - 1) / kTeams)), block(kTeams * kTeam);
+    const bool joint = a.joint != nullptr;
+    void (*k)(McsArgs, TeamLayout) = nullptr;
+#define MUCA_TEAM_PICK(LL, LH)                                                  \
+    k = joint ? k_mcs_team<R, FP64, KEYING, LL, LH, true, NR>                   \
+              : k_mcs_team<R, FP64, KEYING, LL, LH, false, NR>;
+    if (lay.lat_lds && lay.lds_hist) {
+        MUCA_TEAM_PICK(true, true)
+    } else if (lay.lat_lds) {
+        MUCA_TEAM_PICK(true, false)
+    } else if (lay.lds_hist) {
+        MUCA_TEAM_PICK(false, true)
+    } else {
+        MUCA_TEAM_PICK(false, false)
+    }
+#undef MUCA_TEAM_PICK
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lay.bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, block, lay.bytes, st, a, lay);
+    return hipGetLastError();
+}
+
+template <int R, bool FP64, int KEYING>
+static hipError_t launch_team_r(hipStream_t st, const McsArgs &a, const TeamLayout &lay)
+{
+    const uint32_t rounds = (lay.row + 15) / 16;
+    if (rounds <= 1) return launch_team_nr<R, FP64, KEYING, 1>(st, a, lay);
+    if (rounds <= 2) return launch_team_nr<R, FP64, KEYING, 2>(st, a, lay);
+    if (rounds <= 3) return launch_team_nr<R, FP64, KEYING, 3>(st, a, lay);
+    if (rounds <= 4) return launch_team_nr<R, FP64, KEYING, 4>(st, a, lay);
+    return launch_team_nr<R, FP64, KEYING, 8>(st, a, lay);
+}
+
+hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
+                           const TeamLayout &lay)
+{
+    if (cfg.rounds == 10) {
+        if (cfg.fp64)
+            return cfg.keying ? launch_team_r<10, true, 1>(st, a, lay)
+                              : launch_team_r<10, true, 0>(st, a, lay);
+        return cfg.keying ? launch_team_r<10, false, 1>(st, a, lay)
+                          : launch_team_r<10, false, 0>(st, a, lay);
+    }
+    if (cfg.fp64)
+        return cfg.keying ? launch_team_r<7, true, 1>(st, a, lay)
+                          : launch_team_r<7, true, 0>(st, a, lay);
+    return cfg.keying ? launch_team_r<7, false, 1>(st, a, lay)
+                      : launch_team_r<7, false, 0>(st, a, lay);
+}
+
+}  // namespace muca

@@ -59,15 +59,21 @@ CASES = [
 ]
 
 
+KERNELS = pytest.mark.parametrize("kernel", [M.KERNEL_GENERIC, M.KERNEL_TEAM],
+                                  ids=["generic", "team"])
+
+
+@KERNELS
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "L%d_W%d_p%d_k%d_b%g_r%d_t%d_n%d" % c)
-def test_single_launch_parity(orc, case):
+def test_single_launch_parity(orc, case, kernel):
     L, W, prec, keying, beta, rounds, therm, nup = case
     N = L * L
     rng = np.random.default_rng(L * 1000 + W)
     lnw = np.cumsum(rng.normal(scale=1.5, size=N + 1))
     lnw -= lnw[0]
     lat, E, D, Mg = oracle_init(orc, keying, L, W, seed=77, rounds=rounds)
-    with M.Context(L, W, keying=keying, precision=prec, beta=beta, rounds=rounds) as ctx:
+    with M.Context(L, W, keying=keying, precision=prec, beta=beta, rounds=rounds,
+                   kernel=kernel) as ctx:
         ctx.set_lattice(lat.T)
         ctx.set_state(E, D, Mg)
         ctx.set_log_weight(lnw)
@@ -95,10 +101,10 @@ def test_zero_attempts_leaves_state(orc):
         assert int(hist.sum()) == 0
 
 
-def _weight_loop_parity(orc, L, W, iters, seed, recursion=False):
+def _weight_loop_parity(orc, L, W, iters, seed, recursion=False, kernel=M.KERNEL_AUTO):
     recs = orc.weight_iteration_reference(L, W, iters, seed=seed, recursion=recursion,
                                           nthreads=NTHREADS)
-    with M.Context(L, W) as ctx:
+    with M.Context(L, W, kernel=kernel) as ctx:
         ctx.init_lattice(mm=0, gpu=0, seed=seed)
         ctx.compute_state()
         got = []
@@ -114,27 +120,32 @@ def _weight_loop_parity(orc, L, W, iters, seed, recursion=False):
         assert np.array_equal(g["lnw"], r["lnw"]), g["k"]
 
 
-def test_weight_loop_L16_16walkers_10iters(orc):
+@KERNELS
+def test_weight_loop_L16_16walkers_10iters(orc, kernel):
     """Plain method, small: L=16, 16 walkers, seed 1234, 10 trivial iterations."""
-    _weight_loop_parity(orc, 16, 16, 10, 1234)
+    _weight_loop_parity(orc, 16, 16, 10, 1234, kernel=kernel)
 
 
-def test_weight_loop_recursive_L16(orc):
-    _weight_loop_parity(orc, 16, 64, 12, 1234, recursion=True)
+@KERNELS
+def test_weight_loop_recursive_L16(orc, kernel):
+    _weight_loop_parity(orc, 16, 64, 12, 1234, recursion=True, kernel=kernel)
 
 
-def test_weight_loop_L72_4096walkers_20iters(orc):
+@KERNELS
+def test_weight_loop_L72_4096walkers_20iters(orc, kernel):
     """Plain method at reference scale: L=72, 4096 walkers, 20 iterations."""
-    _weight_loop_parity(orc, 72, 4096, 20, 1234)
+    _weight_loop_parity(orc, 72, 4096, 20, 1234, kernel=kernel)
 
 
-def test_produce_joint_histograms_per_block(orc):
+@KERNELS
+def test_produce_joint_histograms_per_block(orc, kernel):
     L, W, B = 8, 96, 3
     N = L * L
     rng = np.random.default_rng(5)
     lnw = np.cumsum(rng.normal(size=N + 1))
     lat, E, D, Mg = oracle_init(orc, M.KEYING_PRODUCE, L, W, seed=100)
-    with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B) as ctx:
+    with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B,
+                   kernel=kernel) as ctx:
         ctx.set_lattice(lat.T)
         ctx.set_state(E, D, Mg)
         ctx.set_log_weight(lnw)
