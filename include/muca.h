@@ -43,6 +43,7 @@ extern "C" {
 #define MUCA_KERNEL_AUTO 0
 #define MUCA_KERNEL_GENERIC 1  /* one lane per walker, reference site order from tables */
 #define MUCA_KERNEL_TEAM 2     /* sixteen lanes per walker, LDS-resident lattices (L % 8 == 0) */
+#define MUCA_KERNEL_LANE 3     /* one lane per walker, even L, table-free, screened acceptance */
 
 typedef struct muca_ctx muca_ctx;
 
@@ -57,6 +58,7 @@ typedef struct muca_params {
     uint32_t joint_blocks; /* produce path: 0 = no joint H(D,M); B >= 1 = one joint
                               histogram per contiguous block of walkers (jackknife) */
     int32_t kernel;      /* MUCA_KERNEL_* */
+    uint32_t block;      /* workgroup size of the one-lane kernels (0: 256; 64..1024) */
 } muca_params;
 
 typedef struct muca_info {
@@ -104,6 +106,14 @@ int muca_set_log_weight(muca_ctx *ctx, const double *lnw, uint32_t nbins);
  * accumulated too (read it with muca_get_joint). */
 int muca_run(muca_ctx *ctx, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t nupdates,
              uint64_t *hist_out, float *kernel_ms);
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * 256 threads per block, muca_delta_multigpu.cu:27): every eligible
+ * (kernel, workgroup size) candidate runs `attempts` attempts per walker on a
+ * scratch copy of the walkers; the fastest becomes the context's choice.
+ * The walkers' lattice, E, D and M are left as they were; the last-launch
+ * histogram is overwritten.  kernel/block/ms (nullable) report the choice. */
+int muca_autotune(muca_ctx *ctx, uint64_t attempts, int32_t *kernel, uint32_t *block,
+                  float *ms);
 /* D2H of the last launch's D histogram (muca_delta_multigpu.cu:401-412);
  * copy_ms (nullable) receives the copy's duration */
 int muca_get_hist(muca_ctx *ctx, uint64_t *hist, uint32_t nbins, float *copy_ms);
@@ -140,6 +150,9 @@ float muca_acclog_host(uint32_t r);
 int muca_acclog_host_range(uint32_t k0, uint32_t n, float *out);
 /* the same function evaluated on the device for r = k << 9, k in [k0, k0+n) */
 int muca_acclog_device(int32_t device, uint32_t k0, uint32_t n, float *out);
+/* the fast screening approximation (hardware log2 * ln 2) the update kernels
+ * use before deciding whether the exact log is needed, same inputs */
+int muca_acclog_fast_device(int32_t device, uint32_t k0, uint32_t n, float *out);
 
 #ifdef __cplusplus
 }

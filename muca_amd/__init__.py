@@ -21,7 +21,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 
 REF32, FP64 = 0, 1
 KEYING_WEIGHT, KEYING_PRODUCE = 0, 1
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_TEAM = 0, 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_TEAM, KERNEL_LANE = 0, 1, 2, 3
 SEED = 100                       # muca_delta_multigpu.cu:19
 
 _u8p = ctypes.POINTER(ctypes.c_int8)
@@ -40,7 +40,7 @@ class Params(ctypes.Structure):
                 ("device", ctypes.c_int32), ("rounds", ctypes.c_uint32),
                 ("beta", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("keying", ctypes.c_int32), ("joint_blocks", ctypes.c_uint32),
-                ("kernel", ctypes.c_int32)]
+                ("kernel", ctypes.c_int32), ("block", ctypes.c_uint32)]
 
 
 class Info(ctypes.Structure):
@@ -70,6 +70,8 @@ _SIGS = {
     "muca_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint64, ctypes.c_uint64, _u64p, _f32p]),
     "muca_get_hist": (ctypes.c_int, [ctypes.c_void_p, _u64p, ctypes.c_uint32, _f32p]),
+    "muca_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _i32p,
+                                     ctypes.POINTER(ctypes.c_uint32), _f32p]),
     "muca_get_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
                                       ctypes.c_uint64]),
     "muca_hist_range": (ctypes.c_int, [_u64p, ctypes.c_uint32, _i32p, _i32p]),
@@ -85,7 +87,12 @@ _SIGS = {
     "muca_acclog_host_range": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _f32p]),
     "muca_acclog_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32,
                                           _f32p]),
+    "muca_acclog_fast_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32,
+                                               ctypes.c_uint32, _f32p]),
 }
+
+ACCLOG_FAST_REL = 2.0 ** -20     # screening band of the fast log (acclog.h)
+ACCLOG_FAST_ABS = 2.0 ** -36
 
 _lib = None
 
@@ -125,9 +132,10 @@ class Context:
     main(), muca_delta_multigpu.cu:177-205 / produce2_multigpu.cu:247-275)."""
 
     def __init__(self, L, walkers=0, device=0, *, rounds=7, beta=2.0, precision=REF32,
-                 keying=KEYING_WEIGHT, joint_blocks=0, kernel=KERNEL_AUTO):
+                 keying=KEYING_WEIGHT, joint_blocks=0, kernel=KERNEL_AUTO, block=0):
         self._h = ctypes.c_void_p()
-        p = Params(L, walkers, device, rounds, beta, precision, keying, joint_blocks, kernel)
+        p = Params(L, walkers, device, rounds, beta, precision, keying, joint_blocks, kernel,
+                   block)
         _check(lib().muca_create(ctypes.byref(p), ctypes.byref(self._h)))
         info = Info()
         _check(lib().muca_query(self._h, ctypes.byref(info)))
@@ -189,6 +197,15 @@ class Context:
         _check(lib().muca_run(self._h, iter & 0xFFFFFFFF, seed, therm, nupdates, _p(h, _u64p),
                               ctypes.byref(ms)))
         return h, ms.value
+
+    def autotune(self, attempts):
+        """Time every eligible (kernel, workgroup) on a scratch copy of the
+        walkers and keep the fastest; returns (kernel, block, ms)."""
+        k, b, ms = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().muca_autotune(self._h, attempts, ctypes.byref(k), ctypes.byref(b),
+                                   ctypes.byref(ms)))
+        self.kernel, self.block = k.value, b.value
+        return k.value, b.value, ms.value
 
     def get_joint(self, block=0):
         n = (self.N + 1) * (self.N + 2) // 2
@@ -253,6 +270,12 @@ def acclog_host_range(k0, n):
 def acclog_device(k0, n, device=0):
     out = np.empty(n, np.float32)
     _check(lib().muca_acclog_device(device, k0, n, _p(out, _f32p)))
+    return out
+
+
+def acclog_fast_device(k0, n, device=0):
+    out = np.empty(n, np.float32)
+    _check(lib().muca_acclog_fast_device(device, k0, n, _p(out, _f32p)))
     return out
 
 

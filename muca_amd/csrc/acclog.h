@@ -87,6 +87,29 @@ __host__ __device__ static inline float acclog_u01(uint32_t r)
     return (float)acclog_u01_double(r);
 }
 
+// ---------------------------------------------------------------------------
+// Fast screening of the acceptance test.  The reference accepts iff
+// diff >= 0 or log(u) < diff; since log(u) < 0 for every u01fixedpt value
+// that is exactly "log(u) < diff".  la = v_log_f32(u) * ln2 differs from the
+// correctly rounded log by at most ACCLOG_FAST_REL * |la| + ACCLOG_FAST_ABS
+// for every one of the 2^23 inputs (half of the band below; checked
+// exhaustively on the device by the test suite).  So
+//   la + band <  diff  => accept,   la - band >= diff  => reject,
+// and only a diff inside the band needs the exact log.
+#define ACCLOG_FAST_REL 0x1p-20f
+#define ACCLOG_FAST_ABS 0x1p-36f
+
+__device__ static inline float acclog_fast_u01(uint32_t r)
+{
+    const float u = (float)(1u | (r >> 8)) * 0x1p-24f;
+    return __builtin_amdgcn_logf(u) * 0x1.62e430p-1f;
+}
+
+__device__ static inline float acclog_fast_band(float la)
+{
+    return fabsf(la) * ACCLOG_FAST_REL + ACCLOG_FAST_ABS;
+}
+
 // device: copy the 128-entry table into LDS (invc at dst[0..127], logc at dst[128..255])
 __device__ static inline void acclog_table_to_lds(double *dst, int tid, int nthreads)
 {

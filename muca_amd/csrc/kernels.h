@@ -58,7 +58,12 @@ struct TeamLayout {
 bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay);
 hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
                            const TeamLayout &lay);
+
+// lane kernel (muca_lane.hip): even L >= 4 and ln W fits in LDS
+bool lane_eligible(const McsArgs &a, int fp64, size_t lds_max);
+hipError_t launch_mcs_lane(hipStream_t st, const Launch &cfg, const McsArgs &a);
 hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out);
+hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 
 // joint H(D,M) -> per-D moments (S0, S1, S2, S4) for each block, FP64
 hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t N,

@@ -166,15 +166,30 @@ int muca_create(const muca_params *params, muca_ctx **out)
     // 6.7 for one lane per walker); one lane per walker wins once there are
     // enough wavefronts to hide its memory latency (58/ns at 65536 walkers
     // against 28).  The crossover sits near 128 walkers per compute unit.
+    // Beyond that the lane kernel leads (213 against 147 at 524288 walkers),
+    // with 512-lane workgroups once there are enough of them to fill every
+    // compute unit twice (279 at 524288 walkers; 256 lanes stay ahead at
+    // 65536).  muca_autotune replaces this rule by measurement.
+    const uint64_t ncu = (uint64_t)(cu > 0 ? cu : 256);
     if (p.kernel == MUCA_KERNEL_AUTO) {
         const bool team_ok = p.L % 8 == 0 && p.L <= 256;
-        c->cfg.kernel = (team_ok && (uint64_t)W <= 128ull * (uint64_t)(cu > 0 ? cu : 256))
-                            ? MUCA_KERNEL_TEAM
-                            : MUCA_KERNEL_GENERIC;
+        const bool lane_ok = p.L % 2 == 0 && p.L >= 4;
+        if (team_ok && (uint64_t)W <= 128ull * ncu)
+            c->cfg.kernel = MUCA_KERNEL_TEAM;
+        else if (lane_ok)
+            c->cfg.kernel = MUCA_KERNEL_LANE;
+        else
+            c->cfg.kernel = MUCA_KERNEL_GENERIC;
     } else {
         c->cfg.kernel = p.kernel;
     }
-    c->cfg.block = 256;
+    if (p.block != 0 && (p.block < 64 || p.block > 1024 || p.block % 64 != 0)) {
+        delete c;
+        return fail(MUCA_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
+    }
+    c->cfg.block = p.block ? (int)p.block : 256;
+    if (p.block == 0 && c->cfg.kernel == MUCA_KERNEL_LANE && (uint64_t)W >= 1024ull * ncu)
+        c->cfg.block = 512;
     c->cfg.lds_max = (size_t)(lds > 0 ? lds : 65536);
     const uint64_t N = c->N;
     int rc = MUCA_OK;
@@ -376,6 +391,105 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     return MUCA_OK;
 }
 
+int muca_autotune(muca_ctx *c, uint64_t attempts, int32_t *kernel, uint32_t *block, float *ms)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    if (!c->have_weights) return fail(MUCA_E_STATE, "muca_set_log_weight was not called");
+    if (attempts == 0) return fail(MUCA_E_ARG, "attempts must be > 0");
+    HIP_TRY(hipSetDevice(c->p.device));
+    const uint64_t N = c->N, W = c->W;
+    int8_t *lat0 = nullptr;
+    int32_t *st0 = nullptr;
+    int rc = dev_alloc(&lat0, N * W, "autotune lattice copy");
+    if (rc) return rc;
+    rc = dev_alloc(&st0, 3 * W, "autotune state copy");
+    if (rc) {
+        hipFree(lat0);
+        return rc;
+    }
+    auto save_restore = [&](bool save) -> hipError_t {
+        hipError_t e;
+        if (save) {
+            e = hipMemcpyAsync(lat0, c->lat, N * W, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess) e = hipMemcpyAsync(st0, c->E, W * 4, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(st0 + W, c->D, W * 4, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(st0 + 2 * W, c->M, W * 4, hipMemcpyDeviceToDevice, c->st);
+        } else {
+            e = hipMemcpyAsync(c->lat, lat0, N * W, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->E, st0, W * 4, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(c->D, st0 + W, W * 4, hipMemcpyDeviceToDevice, c->st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(c->M, st0 + 2 * W, W * 4, hipMemcpyDeviceToDevice, c->st);
+        }
+        return e;
+    };
+    muca::McsArgs a{};
+    a.lat = c->lat;
+    a.E = c->E;
+    a.D = c->D;
+    a.M = c->M;
+    a.lnw32 = c->lnw32;
+    a.lnw64 = c->lnw64;
+    a.order = c->order;
+    a.naver = c->naver;
+    a.hist = c->hist;
+    a.W = W;
+    a.N = c->N;
+    a.L = c->p.L;
+    a.iter = 0xFFFFFFFFu;
+    a.seed = 0;
+    a.therm = attempts - attempts / 2;
+    a.nup = attempts / 2;
+    a.beta = c->p.beta;
+    a.block_walkers = W;
+    struct Cand {
+        int kernel, block;
+    };
+    std::vector<Cand> cands;
+    muca::TeamLayout lay{};
+    if (muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay)) cands.push_back({MUCA_KERNEL_TEAM, 256});
+    if (muca::lane_eligible(a, c->cfg.fp64, c->cfg.lds_max))
+        for (int b : {256, 512, 1024}) cands.push_back({MUCA_KERNEL_LANE, b});
+    cands.push_back({MUCA_KERNEL_GENERIC, 256});
+    hipError_t e = save_restore(true);
+    float best = 0;
+    Cand pick = {c->cfg.kernel, c->cfg.block};
+    for (const Cand &cd : cands) {
+        muca::Launch cfg = c->cfg;
+        cfg.kernel = cd.kernel;
+        cfg.block = cd.block;
+        float t = 0;
+        for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
+            e = save_restore(false);
+            if (e == hipSuccess) e = hipMemsetAsync(c->hist, 0, (N + 1) * 8, c->st);
+            if (e == hipSuccess) e = hipEventRecord(c->ev0, c->st);
+            if (e == hipSuccess) e = muca::launch_mcs(c->st, cfg, a);
+            if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);
+            if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev0, c->ev1);
+        }
+        if (e != hipSuccess) break;
+        if (best == 0 || t < best) {
+            best = t;
+            pick = cd;
+        }
+    }
+    if (e == hipSuccess) e = save_restore(false);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    hipFree(lat0);
+    hipFree(st0);
+    if (e != hipSuccess) return fail(MUCA_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    c->cfg.kernel = pick.kernel;
+    c->cfg.block = pick.block;
+    if (kernel) *kernel = pick.kernel;
+    if (block) *block = (uint32_t)pick.block;
+    if (ms) *ms = best;
+    return MUCA_OK;
+}
+
 int muca_get_hist(muca_ctx *c, uint64_t *hist, uint32_t nbins, float *copy_ms)
 {
     if (!c || !hist) return fail(MUCA_E_ARG, "NULL argument");
@@ -556,6 +670,21 @@ int muca_acclog_device(int32_t device, uint32_t k0, uint32_t n, float *out)
     if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * 4, hipMemcpyDeviceToHost);
     hipFree(d);
     if (e != hipSuccess) return fail(MUCA_E_HIP, std::string("acclog: ") + hipGetErrorString(e));
+    return MUCA_OK;
+}
+
+int muca_acclog_fast_device(int32_t device, uint32_t k0, uint32_t n, float *out)
+{
+    if (!out || n == 0) return fail(MUCA_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(device));
+    float *d = nullptr;
+    int rc = dev_alloc(&d, n, "acclog_fast");
+    if (rc) return rc;
+    hipError_t e = muca::launch_acclog_fast(nullptr, k0, n, d);
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * 4, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess)
+        return fail(MUCA_E_HIP, std::string("acclog_fast: ") + hipGetErrorString(e));
     return MUCA_OK;
 }
 

@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
 template <int R, bool FP64, int KEYING>
 static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
-    const int block = cfg.block > 0 ? cfg.block : 256;
+    const int block = (cfg.block > 0 && cfg.block <= 256) ? cfg.block : 256;   // launch bounds
     const dim3 grid((unsigned)((a.W + block - 1) / block));
     const size_t lds = (size_t)(a.N + 1) * sizeof(uint32_t);
     // uint32 LDS bins cannot overflow: a workgroup adds at most block * nup
@@ -252,6 +252,7 @@ hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a)
         TeamLayout lay{};
         if (team_layout(a, cfg.fp64, cfg.lds_max, lay)) return launch_mcs_team(st, cfg, a, lay);
     }
+    if (cfg.kernel == 3 && lane_eligible(a, cfg.fp64, cfg.lds_max)) return launch_mcs_lane(st, cfg, a);
     if (cfg.fp64)
         return cfg.keying == 0 ? launch_generic_k<true, 0>(st, cfg, a)
                                : launch_generic_k<true, 1>(st, cfg, a);
@@ -271,6 +272,18 @@ __global__ void k_acclog(uint32_t k0, uint32_t n, float *out)
 hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out)
 {
     hipLaunchKernelGGL(k_acclog, dim3((n + 255) / 256), dim3(256), 0, st, k0, n, out);
+    return hipGetLastError();
+}
+
+__global__ void k_acclog_fast(uint32_t k0, uint32_t n, float *out)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = acclog_fast_u01((k0 + t) << 9);
+}
+
+hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *out)
+{
+    hipLaunchKernelGGL(k_acclog_fast, dim3((n + 255) / 256), dim3(256), 0, st, k0, n, out);
     return hipGetLastError();
 }
 

@@ -31,6 +31,24 @@ def test_acclog_device_exhaustive(orc):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k0
 
 
+def test_acclog_fast_band_exhaustive(orc):
+    """The update kernels screen each acceptance with la = v_log_f32(u) * ln 2
+    and fall back to the exact log only when diff lies within
+    band = |la| * 2^-20 + 2^-36 of la.  That is exact iff |la - log(u)| <
+    band for every input; require half the band on all 2^23 inputs."""
+    n, step = 1 << 23, 1 << 22
+    worst = 0.0
+    for k0 in range(0, n, step):
+        la = M.acclog_fast_device(k0, step)
+        ex = orc.logf_u01_range(k0, step)
+        band = (np.abs(la) * np.float32(M.ACCLOG_FAST_REL) +
+                np.float32(M.ACCLOG_FAST_ABS)).astype(np.float64)
+        err = np.abs(la.astype(np.float64) - ex.astype(np.float64))
+        worst = max(worst, float(np.max(err / band)))
+    print(f"worst |la - log u| / band = {worst:.4g}")
+    assert worst <= 0.5
+
+
 @pytest.mark.parametrize("keying", [M.KEYING_WEIGHT, M.KEYING_PRODUCE])
 @pytest.mark.parametrize("L,W,mm,gpu,seed,rounds", [(16, 16, 0, 0, 1234, 7),
                                                      (10, 300, 2, 1, 100, 7),
@@ -59,8 +77,8 @@ CASES = [
 ]
 
 
-KERNELS = pytest.mark.parametrize("kernel", [M.KERNEL_GENERIC, M.KERNEL_TEAM],
-                                  ids=["generic", "team"])
+KERNELS = pytest.mark.parametrize("kernel", [M.KERNEL_GENERIC, M.KERNEL_TEAM, M.KERNEL_LANE],
+                                  ids=["generic", "team", "lane"])
 
 
 @KERNELS
@@ -87,6 +105,28 @@ def test_single_launch_parity(orc, case, kernel):
     assert np.array_equal(gE, E) and np.array_equal(gD, D)
     assert np.array_equal(hist, h)
     assert int(hist.sum()) == W * nup
+
+
+def test_autotune_keeps_state_and_results():
+    """muca_autotune times the candidates on a scratch copy: the walkers are
+    untouched and later launches equal those of a fixed-kernel context."""
+    L, W = 16, 2048
+    N = L * L
+    lnw = -0.4 * np.arange(N + 1) + np.sin(np.arange(N + 1))
+    with M.Context(L, W) as a, M.Context(L, W, kernel=M.KERNEL_GENERIC) as b:
+        for c in (a, b):
+            c.init_lattice(seed=3)
+            c.compute_state()
+            c.set_log_weight(lnw)
+        lat0, st0 = a.get_lattice(), a.get_state()
+        k, blk, ms = a.autotune(2 * N)
+        assert k in (M.KERNEL_GENERIC, M.KERNEL_TEAM, M.KERNEL_LANE) and ms > 0
+        assert np.array_equal(a.get_lattice(), lat0)
+        assert all(np.array_equal(x, y) for x, y in zip(a.get_state(), st0))
+        ha, _ = a.run(5, 77, N, 3 * N + 1)
+        hb, _ = b.run(5, 77, N, 3 * N + 1)
+        assert np.array_equal(ha, hb)
+        assert np.array_equal(a.get_lattice(), b.get_lattice())
 
 
 def test_zero_attempts_leaves_state(orc):
