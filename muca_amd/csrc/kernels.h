@@ -50,6 +50,7 @@ struct TeamLayout {
     uint32_t team_bytes;  // per-team Philox words, attempt records and D sequence
     uint32_t off_rec, off_dseq;  // offsets inside a team's area
     uint32_t off_team, off_tab, off_w, off_hist, off_lat;
+    uint32_t lat_stride;  // LDS bytes per walker lattice (>= N, bank-spread)
     size_t bytes;         // dynamic LDS per workgroup
     bool lds_hist, lat_lds;
 };

@@ -65,14 +65,17 @@ hipError_t launch_init_lattice(hipStream_t st, int8_t *lat, uint64_t W, uint32_t
 // ---------------------------------------------------------------------------
 // per-walker E_J, D, M (muca_delta_multigpu.cu:69-83,548-571; produce2_multigpu.cu:66-89)
 // ---------------------------------------------------------------------------
+// sixteen lanes per walker: lane l sums sites l, l+16, ...; the row of
+// sixteen lanes is then reduced (integer sums, so the order does not matter)
 __global__ __launch_bounds__(256) void k_state(const int8_t *lat, uint64_t W, uint32_t N,
                                                const uint32_t *naver, int32_t *E, int32_t *D,
                                                int32_t *M)
 {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= W) return;
+    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const uint32_t tl = threadIdx.x & 15;
+    if (w >= W) return;   // whole rows of 16 lanes share w
     int32_t e = 0, d = 0, m = 0;
-    for (uint32_t i = 0; i < N; i++) {
+    for (uint32_t i = tl; i < N; i += 16) {
         const int32_t s = lat[(uint64_t)i * W + w];
         const uint32_t *nv = naver + 4u * i;
         const int32_t nb = lat[(uint64_t)nv[0] * W + w] + lat[(uint64_t)nv[1] * W + w] +
@@ -81,16 +84,23 @@ __global__ __launch_bounds__(256) void k_state(const int8_t *lat, uint64_t W, ui
         d += s * s;
         m += s;
     }
-    E[w] = e >> 1;
-    D[w] = d;
-    if (M) M[w] = m;
+    for (int off = 8; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 16);
+        d += __shfl_xor(d, off, 16);
+        m += __shfl_xor(m, off, 16);
+    }
+    if (tl == 0) {
+        E[w] = e >> 1;
+        D[w] = d;
+        if (M) M[w] = m;
+    }
 }
 
 hipError_t launch_state(hipStream_t st, const int8_t *lat, uint64_t W, uint32_t N,
                         const uint32_t *naver, int32_t *E, int32_t *D, int32_t *M)
 {
-    hipLaunchKernelGGL(k_state, dim3((unsigned)((W + 255) / 256)), dim3(256), 0, st, lat, W, N,
-                       naver, E, D, M);
+    hipLaunchKernelGGL(k_state, dim3((unsigned)((W * 16 + 255) / 256)), dim3(256), 0, st, lat, W,
+                       N, naver, E, D, M);
     return hipGetLastError();
 }
 

@@ -102,14 +102,15 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
     if (LATLDS) {
         for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
             const uint32_t site = e >> 4, t = e & 15;
-            latl[e] = (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
+            latl[t * lay.lat_stride + site] =
+                (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
         }
     }
     __syncthreads();
 
     if (active) {
         auto sp = [&](uint32_t site) -> int8_t & {
-            return LATLDS ? latl[site * kTeams + team] : a.lat[(uint64_t)site * W + w];
+            return LATLDS ? latl[team * lay.lat_stride + site] : a.lat[(uint64_t)site * W + w];
         };
         const uint32_t wid = a.w_offset + (uint32_t)w;
         const uint32_t k1a = KEYING == 0 ? 2u * wid : wid;
@@ -120,6 +121,7 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
         uint64_t *jbase = nullptr;
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
         uint32_t c0base = 0;
+        int32_t de_lane = 0, dm_lane = 0;
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             for (uint64_t rs = 0; rs < n; rs += Rw) {
@@ -149,37 +151,42 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
                 const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
                 uint32_t mask[NR];
                 int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
+                // Branch-free over the rounds so that the loads of all rounds
+                // can be in flight together: a lane past the end of the
+                // half-row works on a clamped (in-bounds) attempt and keeps
+                // nothing of it.
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     const uint32_t j = 16 * r + tl;
-                    mask[r] = 0;
-                    info[r] = (8 << 2) | (2 << 7);
-                    if (j < cnt) {
-                        const uint32_t x = 2 * j + h;
-                        const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
-                        const uint32_t xl = (x == 0) ? L - 1 : x - 1;
-                        const int32_t old_s = sp(y * L + x);
-                        const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
-                                           sp(yd * L + x);
-                        const uint32_t r1 = rng[j], r3 = rng[Rw + j];
-                        int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
-                        new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
-                        const int32_t ds = new_s - old_s;
-                        const int32_t dE = -ds * nb;
-                        const int32_t ddel = new_s * new_s - old_s * old_s;
-                        const T fde = (T)(-(double)dE * a.beta);
-                        const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
-                        const T tlog = (T)logu;
-                        uint32_t mk = 0;
+                    const bool valid = j < cnt;
+                    const uint32_t jc = valid ? j : cnt - 1;
+                    const uint32_t x = 2 * jc + h;
+                    const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
+                    const uint32_t xl = (x == 0) ? L - 1 : x - 1;
+                    const int32_t old_s = sp(y * L + x);
+                    const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
+                                       sp(yd * L + x);
+                    const uint32_t r1 = rng[jc], r3 = rng[Rw + jc];
+                    int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
+                    new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
+                    const int32_t ds = new_s - old_s;
+                    const int32_t dE = -ds * nb;
+                    const int32_t ddel = new_s * new_s - old_s * old_s;
+                    const T fde = (T)(-(double)dE * a.beta);
+                    const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
+                    const T tlog = (T)logu;
+                    uint32_t mk = 0;
 #pragma unroll
-                        for (int k = kWin - 1; k >= 0; k--) {
-                            const T wa = ddel > 0 ? win[k + 2] : (ddel < 0 ? win[k] : win[k + 1]);
-                            T diff = fde + wa;
-                            diff = diff - win[k + 1];
-                            mk = mk + mk + (tlog < diff ? 1u : 0u);
-                        }
-                        mask[r] = mk;
-                        info[r] = (ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7);
+                    for (int k = kWin - 1; k >= 0; k--) {
+                        const T wa = ddel > 0 ? win[k + 2] : (ddel < 0 ? win[k] : win[k + 1]);
+                        T diff = fde + wa;
+                        diff = diff - win[k + 1];
+                        mk = mk + mk + (tlog < diff ? 1u : 0u);
+                    }
+                    mask[r] = mk;
+                    info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
+                                    : ((8 << 2) | (2 << 7));
+                    if (valid) {
                         TeamRec<T> rc;
                         rc.fde = fde;
                         rc.logu = logu;
@@ -269,14 +276,19 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
                         }
                     }
                 }
-                e += team_sum(de);
-                const int32_t dmt = team_sum(dm);
-                if (!JOINT) m += dmt;
+                de_lane += de;
+                dm_lane += dm;
                 if (!LATLDS) __threadfence_block();
                 wave_sync();
             }
             c0base += (uint32_t)((n + 3) / 4);
         }
+        // dE and dM of accepted moves were summed per lane; one reduction
+        // over the team's sixteen lanes per launch.  In joint mode the
+        // chain itself tracks M (it is needed per attempt).
+        e += team_sum(de_lane);
+        const int32_t dm_team = team_sum(dm_lane);
+        if (!JOINT) m += dm_team;
         if (tl == 0) {
             a.D[w] = d;
             a.E[w] = e;
@@ -293,7 +305,7 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
     if (LATLDS) {
         for (uint32_t e2 = tid; e2 < N * kTeams; e2 += blockDim.x) {
             const uint32_t site = e2 >> 4, t = e2 & 15;
-            if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[e2];
+            if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[t * lay.lat_stride + site];
         }
     }
 }
@@ -323,8 +335,14 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     lay.off_hist = (uint32_t)off;
     if (hist_fits) off += al((a.N + 1) * 4);
     lay.off_lat = (uint32_t)off;
-    lay.lat_lds = off + al((size_t)a.N * kTeams) <= lds_max;
-    if (lay.lat_lds) off += al((size_t)a.N * kTeams);
+    // walker-major lattices: the lanes of a team read sites two bytes apart
+    // (one dword per two lanes); a per-walker stride of 8 (mod 32) dwords puts
+    // the four teams of a wavefront on disjoint LDS banks.
+    uint32_t stride = (uint32_t)((a.N + 3) & ~3u);
+    while ((stride / 4) % 32 != 8) stride += 4;
+    lay.lat_stride = stride;
+    lay.lat_lds = off + al((size_t)stride * kTeams) <= lds_max;
+    if (lay.lat_lds) off += al((size_t)stride * kTeams);
     lay.bytes = off;
     return off <= lds_max;
 }
