@@ -143,6 +143,63 @@ int muca_jackknife(const double *x, uint32_t J, double *avg, double *err);
 int muca_reweight(muca_ctx *ctx, const double *deltas, uint32_t ndelta, double *absM,
                   double *chi, double *binder);
 
+/* ---- batched weight iteration on the device ----
+ * Many independent weight problems (each one run of multigpu_weight for one
+ * jack on one GPU: muca_delta_multigpu.cu:314-476) advanced together: one
+ * persistent update launch over every unconverged instance's walkers, then one
+ * device launch doing the host loop body (hist_range, width, kull_leibler,
+ * convergence, trivial or recursive ln W update in FP64, lnW -= lnW[0], the
+ * NUPDATES schedule).  Per instance: weight keying, Philox4x32-7, therm = N
+ * (therm_coeff = 1, :315-316), walker ids 0..walkers-1, the instance's seed,
+ * start width 10 (:314).  The device update uses the device libm log; the
+ * reference's host loop uses the host one. */
+typedef struct muca_batch muca_batch;
+
+typedef struct muca_instance {
+    uint32_t L;          /* even, >= 4 */
+    uint32_t walkers;
+    uint32_t seed;
+    uint32_t init_nup;   /* NUPDATES of the first iteration (0: 1) */
+    double beta;         /* 2.0 reproduces the reference's -dE*2 */
+    int32_t recursion;   /* 0 trivial update, 1 weight_recursion */
+    uint32_t reserved;
+} muca_instance;
+
+typedef struct muca_instance_status {
+    uint32_t iter;       /* iterations done */
+    uint32_t nup;        /* NUPDATES of the next iteration */
+    uint32_t width;      /* widest histogram range seen (:432-433) */
+    uint32_t converged;  /* width == N and Kullback-Leibler < 1e-4 (:447) */
+    uint32_t reach_iter; /* iteration at which width reached N (0xFFFFFFFF: not yet) */
+    uint32_t done_iter;  /* iteration that converged (0xFFFFFFFF: not yet) */
+    int32_t start, end;  /* hist_range of the last iteration */
+    double dkl;          /* Kullback-Leibler divergence of the last iteration */
+    uint64_t attempts;   /* flip attempts done (therm + nupdates per walker) */
+} muca_instance_status;
+
+const char *muca_batch_last_error(void);
+/* tiles: MUCA_KERNEL_AUTO, MUCA_KERNEL_TEAM (every L % 8 == 0, L <= 128) or
+ * MUCA_KERNEL_LANE; precision: MUCA_REF32 | MUCA_FP64 */
+int muca_batch_create(int32_t device, const muca_instance *instances, uint32_t n,
+                      int32_t precision, int32_t tiles, muca_batch **batch);
+int muca_batch_destroy(muca_batch *batch);
+int muca_batch_query(const muca_batch *batch, int32_t *tiles, uint32_t *ntiles, uint32_t *grid,
+                     uint64_t *walkers);
+/* random lattices (weight keying, jack index mm), E/D/M, ln W = 0, counters reset */
+int muca_batch_init(muca_batch *batch, uint32_t mm);
+int muca_batch_set_log_weight(muca_batch *batch, uint32_t instance, const double *lnw,
+                              uint32_t nbins);
+int muca_batch_get_log_weight(muca_batch *batch, uint32_t instance, double *lnw,
+                              uint32_t nbins);
+/* the D histogram of the instance's last iteration */
+int muca_batch_get_hist(muca_batch *batch, uint32_t instance, uint64_t *hist, uint32_t nbins);
+/* `iterations` weight iterations of every unconverged instance; use_graph
+ * replays one captured hipGraph per iteration; gpu_ms (nullable) receives
+ * the elapsed device time */
+int muca_batch_iterate(muca_batch *batch, uint32_t iterations, int32_t use_graph,
+                       float *gpu_ms);
+int muca_batch_status(muca_batch *batch, muca_instance_status *status, uint32_t n);
+
 /* ---- self-checks usable without a GPU ---- */
 /* the acceptance log evaluated by the host instantiation of the device code */
 float muca_acclog_host(uint32_t r);

@@ -52,8 +52,39 @@ class Info(ctypes.Structure):
                 ("lds_lattice", ctypes.c_uint32)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("walkers", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("init_nup", ctypes.c_uint32),
+                ("beta", ctypes.c_double), ("recursion", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class InstanceStatus(ctypes.Structure):
+    _fields_ = [("iter", ctypes.c_uint32), ("nup", ctypes.c_uint32),
+                ("width", ctypes.c_uint32), ("converged", ctypes.c_uint32),
+                ("reach_iter", ctypes.c_uint32), ("done_iter", ctypes.c_uint32),
+                ("start", ctypes.c_int32), ("end", ctypes.c_int32),
+                ("dkl", ctypes.c_double), ("attempts", ctypes.c_uint64)]
+
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+
 # every symbol declared in include/muca.h, with its ctypes signature
 _SIGS = {
+    "muca_batch_last_error": (ctypes.c_char_p, []),
+    "muca_batch_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(Instance), _u32,
+                                         ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "muca_batch_destroy": (ctypes.c_int, [_vp]),
+    "muca_batch_query": (ctypes.c_int, [_vp, _i32p, ctypes.POINTER(_u32),
+                                        ctypes.POINTER(_u32), _u64p]),
+    "muca_batch_init": (ctypes.c_int, [_vp, _u32]),
+    "muca_batch_set_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
+    "muca_batch_get_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
+    "muca_batch_get_hist": (ctypes.c_int, [_vp, _u32, _u64p, _u32]),
+    "muca_batch_iterate": (ctypes.c_int, [_vp, _u32, ctypes.c_int32, _f32p]),
+    "muca_batch_status": (ctypes.c_int, [_vp, ctypes.POINTER(InstanceStatus), _u32]),
     "muca_last_error": (ctypes.c_char_p, []),
     "muca_default_walkers": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32)]),
     "muca_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
@@ -220,6 +251,82 @@ class Context:
         out = [np.empty(shp, np.float64) for _ in range(3)]
         _check(lib().muca_reweight(self._h, _p(d, _f64p), d.size, *[_p(o, _f64p) for o in out]))
         return out
+
+
+def _bcheck(rc):
+    if rc != 0:
+        raise MucaError(f"libmuca batch error {rc}: {lib().muca_batch_last_error().decode()}")
+
+
+class Batch:
+    """Many independent weight problems advanced together on one device (each
+    the weight loop of multigpu_weight for one jack,
+    muca_delta_multigpu.cu:314-476); the loop body runs on the device.
+
+    instances: iterable of dicts with keys L, walkers, seed, and optionally
+    beta (2.0), init_nup (1), recursion (False)."""
+
+    def __init__(self, instances, device=0, *, precision=REF32, tiles=KERNEL_AUTO):
+        ins = list(instances)
+        arr = (Instance * len(ins))()
+        for i, q in enumerate(ins):
+            arr[i] = Instance(q["L"], q["walkers"], q["seed"], q.get("init_nup", 1),
+                              q.get("beta", 2.0), 1 if q.get("recursion") else 0, 0)
+        self._h = ctypes.c_void_p()
+        _bcheck(lib().muca_batch_create(device, arr, len(ins), precision, tiles,
+                                        ctypes.byref(self._h)))
+        self.n = len(ins)
+        self.N = [q["L"] * q["L"] for q in ins]
+        k, nt, g, w = ctypes.c_int32(), _u32(), _u32(), ctypes.c_uint64()
+        _bcheck(lib().muca_batch_query(self._h, ctypes.byref(k), ctypes.byref(nt),
+                                       ctypes.byref(g), ctypes.byref(w)))
+        self.tiles, self.ntiles, self.grid, self.walkers = k.value, nt.value, g.value, w.value
+
+    def close(self):
+        if self._h:
+            lib().muca_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, mm=0):
+        _bcheck(lib().muca_batch_init(self._h, mm))
+
+    def set_log_weight(self, i, lnw):
+        a = np.ascontiguousarray(lnw, np.float64)
+        _bcheck(lib().muca_batch_set_log_weight(self._h, i, _p(a, _f64p), a.size))
+
+    def get_log_weight(self, i):
+        a = np.empty(self.N[i] + 1, np.float64)
+        _bcheck(lib().muca_batch_get_log_weight(self._h, i, _p(a, _f64p), a.size))
+        return a
+
+    def get_hist(self, i):
+        a = np.empty(self.N[i] + 1, np.uint64)
+        _bcheck(lib().muca_batch_get_hist(self._h, i, _p(a, _u64p), a.size))
+        return a
+
+    def iterate(self, iterations=1, use_graph=True):
+        """Returns the elapsed device milliseconds."""
+        ms = ctypes.c_float()
+        _bcheck(lib().muca_batch_iterate(self._h, iterations, 1 if use_graph else 0,
+                                         ctypes.byref(ms)))
+        return ms.value
+
+    def status(self):
+        arr = (InstanceStatus * self.n)()
+        _bcheck(lib().muca_batch_status(self._h, arr, self.n))
+        return [{f: getattr(s, f) for f, _ in InstanceStatus._fields_} for s in arr]
 
 
 # ---- host pieces of the weight iteration (muca_delta_multigpu.cu:431-476) ----

@@ -27,7 +27,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
-LIB_SOURCES = ["muca_kernels.hip", "muca_team.hip", "muca_lane.hip", "muca_api.cpp"]
+LIB_SOURCES = ["muca_kernels.hip", "muca_team.hip", "muca_lane.hip", "muca_batch.hip",
+               "muca_api.cpp"]
 CLI_SOURCES = {"multigpu_weight": "cli_weight.cpp", "produce_gpu": "cli_produce.cpp"}
 
 

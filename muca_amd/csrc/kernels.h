@@ -63,6 +63,85 @@ hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
 // lane kernel (muca_lane.hip): even L >= 4 and ln W fits in LDS
 bool lane_eligible(const McsArgs &a, int fp64, size_t lds_max);
 hipError_t launch_mcs_lane(hipStream_t st, const Launch &cfg, const McsArgs &a);
+
+// ---- batched engine (muca_batch.hip) ----
+// One multicanonical weight problem of a batch.  The first block is set on
+// the host; the second is the per-iteration state the device update kernel
+// advances (the reference's host loop variables, muca_delta_multigpu.cu:314-447).
+struct BatchInst {
+    uint32_t L, N, W, seed;
+    uint64_t lat_off;      // bytes into the batch lattice buffer (site-major per instance)
+    uint64_t st_off;       // walkers into the E/D/M arrays
+    uint64_t w_off;        // bins into ln W / histogram / scratch arrays
+    double beta;
+    int32_t recursion;     // 0 trivial update, 1 weight_recursion
+    uint32_t lane_lds_w;   // lane tiles: LDS bytes of ln W
+    uint32_t lane_hist;    // lane tiles: histogram in LDS
+    TeamLayout lay;        // team tiles
+    // advanced on the device
+    uint32_t iter, nup, width, reach, converged, reach_iter, done_iter;
+    int32_t start, end;
+    double dkl;
+    uint64_t attempts;     // flip attempts done so far
+};
+
+struct BatchTile {
+    uint32_t inst, wbase;
+};
+
+struct BatchArgs {
+    BatchInst *inst;
+    const BatchTile *tiles;
+    uint32_t ntiles, ninst;
+    uint32_t *queue;       // tile counter of the persistent kernel
+    int8_t *lat;
+    int32_t *E, *D, *M;
+    float *lnw32;
+    double *lnw64;
+    uint64_t *hist, *hist_last;
+    double *bf, *p_acc;    // recursion scratch
+};
+
+// launch arguments of one instance for its current iteration (therm = N,
+// the reference's therm_coeff = 1, muca_delta_multigpu.cu:315-316)
+__device__ __forceinline__ McsArgs batch_mcs_args(const BatchArgs &b, const BatchInst &I)
+{
+    McsArgs a{};
+    a.lat = b.lat + I.lat_off;
+    a.E = b.E + I.st_off;
+    a.D = b.D + I.st_off;
+    a.M = b.M + I.st_off;
+    a.lnw32 = b.lnw32 + I.w_off;
+    a.lnw64 = b.lnw64 + I.w_off;
+    a.order = nullptr;
+    a.naver = nullptr;
+    a.hist = b.hist + I.w_off;
+    a.joint = nullptr;
+    a.W = I.W;
+    a.N = I.N;
+    a.L = I.L;
+    a.w_offset = 0;
+    a.iter = I.iter;
+    a.seed = I.seed;
+    a.therm = I.N;
+    a.nup = I.nup;
+    a.beta = I.beta;
+    a.joint_stride = 0;
+    a.block_walkers = I.W;
+    a.joint_blocks = 0;
+    return a;
+}
+
+// persistent kernels: every workgroup takes tiles from *queue until none is left
+hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
+                             size_t lds_bytes);
+hipError_t launch_batch_lane(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
+                             uint32_t block, size_t lds_bytes);
+// the kernels' dynamic-LDS limit, set once outside any graph capture
+hipError_t prepare_batch_team(int fp64, size_t lds_bytes);
+hipError_t prepare_batch_lane(int fp64, size_t lds_bytes);
+// one workgroup per instance: histogram bookkeeping, convergence, ln W update
+hipError_t launch_batch_update(hipStream_t st, const BatchArgs &b);
 hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 

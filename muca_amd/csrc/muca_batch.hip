@@ -1,0 +1,575 @@
+// muca_batch.hip -- the batched engine: many independent multicanonical
+// weight problems (instances of different L, beta, seed, walker count) advanced
+// together, one weight iteration per two launches:
+//   1. a persistent update kernel (team or lane tiles, muca_team.hip /
+//      muca_lane.hip) that walks every unconverged instance's walkers through
+//      therm = N and NUPDATES attempts into per-instance uint64 histograms;
+//   2. k_batch_update, one workgroup per instance, which does on the device
+//      what the reference's host loop does between launches
+//      (muca_delta_multigpu.cu:431-476): hist_range, the width rule,
+//      kull_leibler, the convergence test, the trivial update or
+//      weight_recursion in FP64, lnW -= lnW[0], the float copy, and the
+//      NUPDATES schedule for the next iteration.
+// Both launches read all per-iteration state from device memory, so one
+// iteration is captured once into a hipGraph and replayed; nothing returns to
+// the host between iterations.  The device update uses the device log, which
+// may differ from the host libm log in the last bit; the update kernels'
+// chains are exact given the weights they are handed.
+#include "../../include/muca.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace muca {
+
+hipError_t prepare_batch_team(int fp64, size_t lds_bytes);
+hipError_t prepare_batch_lane(int fp64, size_t lds_bytes);
+
+namespace {
+
+constexpr int kUpd = 1024;   // threads of the update workgroup
+
+template <typename T, typename Op>
+__device__ T block_reduce(T v, T ident, Op op, T *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    __syncthreads();
+    if (ln == 0) sh[wv] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        v = threadIdx.x < (blockDim.x >> 6) ? sh[threadIdx.x] : ident;
+        for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+        if (threadIdx.x == 0) sh[0] = v;
+    }
+    __syncthreads();
+    const T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+struct OpMin {
+    __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; }
+};
+struct OpMax {
+    __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; }
+};
+struct OpAddU {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; }
+};
+struct OpAddD {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+
+}  // namespace
+
+__global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
+{
+    __shared__ int64_t sh_i[16];
+    __shared__ uint64_t sh_u[16];
+    __shared__ double sh_d[16];
+    __shared__ int s_conv, s_rec;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0;   // tiles of the next iteration
+    BatchInst *I = b.inst + blockIdx.x;
+    const uint32_t N = I->N;
+    uint64_t *h = b.hist + I->w_off;
+    uint64_t *hl = b.hist_last + I->w_off;
+    double *lnw = b.lnw64 + I->w_off;
+    float *l32 = b.lnw32 + I->w_off;
+    const int was_converged = I->converged;
+    if (!was_converged) {
+        // hist_range (muca_delta_multigpu.cu:573-596) and the kull_leibler sums (:598-606)
+        int64_t st = (int64_t)N + 1, en = -1;
+        uint64_t ev = 0, nb = 0;
+        for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+            const uint64_t c = h[i];
+            if (c > 1) {
+                st = st < (int64_t)i ? st : (int64_t)i;
+                en = (int64_t)i;
+            }
+            if (c > 0) {
+                ev += c;
+                nb += 1;
+            }
+        }
+        st = block_reduce(st, (int64_t)N + 1, OpMin(), sh_i);
+        en = block_reduce(en, (int64_t)-1, OpMax(), sh_i);
+        ev = block_reduce(ev, (uint64_t)0, OpAddU(), sh_u);
+        nb = block_reduce(nb, (uint64_t)0, OpAddU(), sh_u);
+        int32_t start = (int32_t)st, end = en < 0 ? 0 : (int32_t)en;
+        if (start > end) start = end = 0;
+        // kull_leibler (:609-615); the sums of integer counts are exact
+        const double num_event = (double)ev, num_bin = (double)nb;
+        double kl = 0;
+        for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+            const uint64_t c = h[i];
+            if (c > 0) {
+                const double P = (double)c / num_event;
+                const double Q = 1.0 / num_bin;
+                kl = kl + P * log(P / Q);
+            }
+        }
+        kl = block_reduce(kl, 0.0, OpAddD(), sh_d);
+        if (threadIdx.x == 0) {
+            const uint32_t wt = (uint32_t)(end - start);            // :432-433
+            if (wt > I->width) I->width = wt;
+            I->start = start;
+            I->end = end;
+            I->dkl = kl;
+            I->attempts += (uint64_t)I->W * ((uint64_t)I->N + I->nup);
+            if (I->width == N && !I->reach) {                       // :441-446
+                I->reach = 1;
+                I->reach_iter = I->iter;
+            }
+            const int conv = (I->width == N) && (kl < 1e-4);        // :447
+            if (conv) {   // the reference leaves the loop before updating (:462)
+                I->converged = 1;
+                I->done_iter = I->iter;
+                I->iter += 1;
+            }
+            s_conv = conv;
+            s_rec = I->recursion;
+        }
+        __syncthreads();
+        if (!s_conv) {
+            if (!s_rec) {
+                // trivial update (:467)
+                for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+                    const uint64_t c = h[i];
+                    if (c > 1) lnw[i] = lnw[i] - log((double)c);
+                }
+            } else {
+                // weight_recursion (:848-858): a sequential prefix over [start, end)
+                double *bf = b.bf + I->w_off, *pacc = b.p_acc + I->w_off;
+                for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) bf[i] = lnw[i];
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    for (int32_t i = start; i < end; i++) {
+                        const uint64_t a0 = h[i], a1 = h[i + 1];
+                        if ((a0 != 0) & (a1 != 0)) {
+                            const double p = (double)(a0 * a1) / (double)(a0 + a1);
+                            const double k = p / (p + pacc[i]);
+                            const double dS = -bf[i] + bf[i + 1] + k * log((double)a0 / (double)a1);
+                            lnw[i + 1] = lnw[i] + dS;
+                            pacc[i] = pacc[i] + p;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const double t0 = lnw[0];                               // :469-475
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+                const double v = lnw[i] - t0;
+                lnw[i] = v;
+                l32[i] = (float)v;
+            }
+            if (threadIdx.x == 0) {
+                if (I->width == N) {                                 // :379-382, next iteration
+                    const uint32_t nu = I->nup + 1;
+                    I->nup = (uint32_t)(nu * 1.01);
+                }
+                I->iter += 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (!was_converged) {
+        for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+            hl[i] = h[i];
+            h[i] = 0;
+        }
+    }
+}
+
+hipError_t launch_batch_update(hipStream_t st, const BatchArgs &b)
+{
+    hipLaunchKernelGGL(k_batch_update, dim3(b.ninst), dim3(kUpd), 0, st, b);
+    return hipGetLastError();
+}
+
+}  // namespace muca
+
+// ---------------------------------------------------------------------------
+// host object and C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_berr;
+
+int bfail(int code, const std::string &m)
+{
+    g_berr = m;
+    return code;
+}
+
+#define BHIP(expr)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return bfail(MUCA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct muca_batch {
+    int device = 0, fp64 = 0, kind = 0;   // kind: MUCA_KERNEL_TEAM | MUCA_KERNEL_LANE
+    uint32_t block = 256, grid = 0, cus = 0;
+    size_t lds_bytes = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<muca::BatchInst> inst;
+    std::vector<uint32_t> init_nup;
+    std::vector<muca::BatchTile> tiles;
+    muca::BatchArgs a{};
+    muca::BatchTile *d_tiles = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t lat_bytes = 0, walkers = 0, bins = 0;
+};
+
+namespace {
+
+void batch_free(muca_batch *b)
+{
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    void *bufs[] = {b->a.inst, b->d_tiles, b->a.queue, b->a.lat, b->a.E, b->a.D, b->a.M,
+                    b->a.lnw32, b->a.lnw64, b->a.hist, b->a.hist_last, b->a.bf, b->a.p_acc};
+    for (void *p : bufs)
+        if (p) hipFree(p);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+}
+
+template <typename T>
+hipError_t balloc(T **p, uint64_t n)
+{
+    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+
+hipError_t launch_iteration(muca_batch *b)
+{
+    hipError_t e = b->kind == MUCA_KERNEL_TEAM
+                       ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes)
+                       : muca::launch_batch_lane(b->st, b->fp64, b->a, b->grid, b->block,
+                                                 b->lds_bytes);
+    if (e != hipSuccess) return e;
+    return muca::launch_batch_update(b->st, b->a);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *muca_batch_last_error(void)
+{
+    return g_berr.c_str();
+}
+
+int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int32_t precision,
+                      int32_t tiles, muca_batch **out)
+{
+    if (!ins || !out || n == 0) return bfail(MUCA_E_ARG, "bad arguments");
+    *out = nullptr;
+    if (precision != MUCA_REF32 && precision != MUCA_FP64)
+        return bfail(MUCA_E_ARG, "precision must be MUCA_REF32 or MUCA_FP64");
+    int ndev = 0;
+    BHIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return bfail(MUCA_E_ARG, "device not present");
+    BHIP(hipSetDevice(device));
+    int cu = 0, lds = 0;
+    BHIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+    BHIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    const size_t lds_budget = (size_t)lds - 256;   // static LDS of the persistent kernels
+    muca_batch *b = new muca_batch();
+    b->device = device;
+    b->fp64 = precision == MUCA_FP64;
+    b->cus = (uint32_t)cu;
+    bool team_ok = true, lane_ok = true;
+    uint64_t lat = 0, st = 0, wo = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const muca_instance &q = ins[i];
+        if (q.L < 4 || q.L % 2 != 0 || q.L > 46340 || q.walkers == 0 || !(q.beta > 0)) {
+            batch_free(b);
+            return bfail(MUCA_E_ARG, "instance " + std::to_string(i) +
+                                         ": need even L >= 4, walkers > 0, beta > 0");
+        }
+        muca::BatchInst I{};
+        I.L = q.L;
+        I.N = q.L * q.L;
+        I.W = q.walkers;
+        I.seed = q.seed;
+        I.beta = q.beta;
+        I.recursion = q.recursion ? 1 : 0;
+        I.lat_off = lat;
+        I.st_off = st;
+        I.w_off = wo;
+        lat += (uint64_t)I.N * I.W;
+        st += I.W;
+        wo += I.N + 1;
+        muca::McsArgs a{};
+        a.N = I.N;
+        a.L = I.L;
+        a.W = I.W;
+        a.nup = 1;
+        if (!(I.L % 8 == 0 && I.L <= 128 && muca::team_layout(a, b->fp64, lds_budget, I.lay)))
+            team_ok = false;
+        const size_t tsz = b->fp64 ? 8 : 4;
+        I.lane_lds_w = (uint32_t)(((I.N + 1) * tsz + 15) & ~(size_t)15);
+        I.lane_hist = I.lane_lds_w + (size_t)(I.N + 1) * 4 <= lds_budget;
+        if (I.lane_lds_w > lds_budget) lane_ok = false;
+        b->inst.push_back(I);
+        b->init_nup.push_back(q.init_nup ? q.init_nup : 1);
+    }
+    b->lat_bytes = lat;
+    b->walkers = st;
+    b->bins = wo;
+    if (tiles == MUCA_KERNEL_TEAM && !team_ok) {
+        batch_free(b);
+        return bfail(MUCA_E_ARG, "team tiles need every L % 8 == 0 and L <= 128");
+    }
+    if (tiles == MUCA_KERNEL_LANE && !lane_ok) {
+        batch_free(b);
+        return bfail(MUCA_E_ARG, "lane tiles need ln W of every instance in LDS");
+    }
+    if (tiles == MUCA_KERNEL_TEAM || tiles == MUCA_KERNEL_LANE)
+        b->kind = tiles;
+    else  // same rule as the single-instance contexts: team while walkers are few
+        b->kind = (team_ok && (st <= 128ull * (uint64_t)cu || !lane_ok)) ? MUCA_KERNEL_TEAM
+                                                                          : MUCA_KERNEL_LANE;
+    if (b->kind == MUCA_KERNEL_LANE && !lane_ok) {
+        batch_free(b);
+        return bfail(MUCA_E_ARG, "no tile kind fits every instance");
+    }
+    // tiles, largest lattices first
+    const uint32_t per = b->kind == MUCA_KERNEL_TEAM ? 16 : b->block;
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return b->inst[x].N > b->inst[y].N; });
+    for (uint32_t i : order)
+        for (uint32_t w = 0; w < b->inst[i].W; w += per) b->tiles.push_back({i, w});
+    size_t need = 0;
+    for (const auto &I : b->inst)
+        need = std::max(need, b->kind == MUCA_KERNEL_TEAM
+                                  ? I.lay.bytes
+                                  : (size_t)I.lane_lds_w + (I.lane_hist ? (size_t)(I.N + 1) * 4 : 0));
+    b->lds_bytes = need;
+    const uint32_t per_cu_lds = (uint32_t)std::max<size_t>(1, (size_t)lds / (need + 256));
+    const uint32_t per_cu_thr = b->kind == MUCA_KERNEL_TEAM ? 8 : 2048 / b->block;
+    b->grid = (uint32_t)cu * std::min(per_cu_lds, per_cu_thr);
+    b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->tiles.size());
+    hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    muca::BatchArgs &A = b->a;
+    A.ntiles = (uint32_t)b->tiles.size();
+    A.ninst = n;
+    if (e == hipSuccess) e = balloc(&A.inst, n);
+    if (e == hipSuccess) e = balloc(&b->d_tiles, b->tiles.size());
+    if (e == hipSuccess) e = balloc(&A.queue, 1);
+    if (e == hipSuccess) e = balloc(&A.lat, lat);
+    if (e == hipSuccess) e = balloc(&A.E, st);
+    if (e == hipSuccess) e = balloc(&A.D, st);
+    if (e == hipSuccess) e = balloc(&A.M, st);
+    if (e == hipSuccess) e = balloc(&A.lnw32, wo);
+    if (e == hipSuccess) e = balloc(&A.lnw64, wo);
+    if (e == hipSuccess) e = balloc(&A.hist, wo);
+    if (e == hipSuccess) e = balloc(&A.hist_last, wo);
+    if (e == hipSuccess) e = balloc(&A.bf, wo);
+    if (e == hipSuccess) e = balloc(&A.p_acc, wo);
+    A.tiles = b->d_tiles;
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(muca::BatchTile),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = b->kind == MUCA_KERNEL_TEAM ? muca::prepare_batch_team(b->fp64, b->lds_bytes)
+                                        : muca::prepare_batch_lane(b->fp64, b->lds_bytes);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return bfail(MUCA_E_NOMEM, std::string("batch setup: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return MUCA_OK;
+}
+
+int muca_batch_destroy(muca_batch *b)
+{
+    batch_free(b);
+    return MUCA_OK;
+}
+
+int muca_batch_query(const muca_batch *b, int32_t *kind, uint32_t *tiles, uint32_t *grid,
+                     uint64_t *walkers)
+{
+    if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    if (kind) *kind = b->kind;
+    if (tiles) *tiles = (uint32_t)b->tiles.size();
+    if (grid) *grid = b->grid;
+    if (walkers) *walkers = b->walkers;
+    return MUCA_OK;
+}
+
+int muca_batch_init(muca_batch *b, uint32_t mm)
+{
+    if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    BHIP(hipSetDevice(b->device));
+    muca::BatchArgs &A = b->a;
+    uint32_t maxN = 0;
+    for (const auto &I : b->inst) maxN = std::max(maxN, I.N);
+    struct DevBuf {
+        uint32_t *p = nullptr;
+        ~DevBuf()
+        {
+            if (p) hipFree(p);
+        }
+    } naver_d;
+    BHIP(hipMalloc((void **)&naver_d.p, 16ull * maxN));
+    std::vector<uint32_t> naver;
+    uint32_t cur_L = 0;
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        muca::BatchInst &I = b->inst[i];
+        BHIP(muca::launch_init_lattice(b->st, A.lat + I.lat_off, I.W, I.N, MUCA_KEYING_WEIGHT, 7,
+                                       mm, 0, I.seed, 0));
+        if (I.L != cur_L) {   // neighbour table of this L (muca_delta_multigpu.cu:157-172)
+            BHIP(hipStreamSynchronize(b->st));   // the previous table may still be read
+            naver.assign(4ull * I.N, 0);
+            for (uint32_t s = 0; s < I.N; s++) {
+                const uint32_t x = s % I.L, y = s / I.L;
+                naver[4 * s + 0] = y * I.L + (x + 1) % I.L;
+                naver[4 * s + 1] = y * I.L + (x + I.L - 1) % I.L;
+                naver[4 * s + 2] = ((y + 1) % I.L) * I.L + x;
+                naver[4 * s + 3] = ((y + I.L - 1) % I.L) * I.L + x;
+            }
+            BHIP(hipMemcpy(naver_d.p, naver.data(), naver.size() * 4, hipMemcpyHostToDevice));
+            cur_L = I.L;
+        }
+        BHIP(muca::launch_state(b->st, A.lat + I.lat_off, I.W, I.N, naver_d.p, A.E + I.st_off,
+                                A.D + I.st_off, A.M + I.st_off));
+        I.iter = 0;
+        I.nup = b->init_nup[i];
+        I.width = 10;   // :314
+        I.reach = I.converged = I.reach_iter = I.done_iter = 0;
+        I.start = I.end = 0;
+        I.dkl = 0;
+        I.attempts = 0;
+    }
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(A.inst, b->inst.data(), b->inst.size() * sizeof(muca::BatchInst),
+                   hipMemcpyHostToDevice));
+    BHIP(hipMemset(A.queue, 0, 4));
+    BHIP(hipMemset(A.lnw32, 0, b->bins * 4));
+    BHIP(hipMemset(A.lnw64, 0, b->bins * 8));
+    BHIP(hipMemset(A.hist, 0, b->bins * 8));
+    BHIP(hipMemset(A.hist_last, 0, b->bins * 8));
+    BHIP(hipMemset(A.p_acc, 0, b->bins * 8));
+    return MUCA_OK;
+}
+
+int muca_batch_set_log_weight(muca_batch *b, uint32_t i, const double *lnw, uint32_t nbins)
+{
+    if (!b || !lnw || i >= b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    const muca::BatchInst &I = b->inst[i];
+    if (nbins != I.N + 1) return bfail(MUCA_E_ARG, "nbins must be N+1");
+    std::vector<float> f(nbins);
+    for (uint32_t k = 0; k < nbins; k++) {
+        if (!std::isfinite(lnw[k])) return bfail(MUCA_E_ARG, "ln W must be finite");
+        f[k] = (float)lnw[k];
+    }
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(b->a.lnw64 + I.w_off, lnw, nbins * 8, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->a.lnw32 + I.w_off, f.data(), nbins * 4, hipMemcpyHostToDevice));
+    return MUCA_OK;
+}
+
+int muca_batch_get_log_weight(muca_batch *b, uint32_t i, double *lnw, uint32_t nbins)
+{
+    if (!b || !lnw || i >= b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    const muca::BatchInst &I = b->inst[i];
+    if (nbins != I.N + 1) return bfail(MUCA_E_ARG, "nbins must be N+1");
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(lnw, b->a.lnw64 + I.w_off, nbins * 8, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_batch_get_hist(muca_batch *b, uint32_t i, uint64_t *hist, uint32_t nbins)
+{
+    if (!b || !hist || i >= b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    const muca::BatchInst &I = b->inst[i];
+    if (nbins != I.N + 1) return bfail(MUCA_E_ARG, "nbins must be N+1");
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(hist, b->a.hist_last + I.w_off, nbins * 8, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_batch_iterate(muca_batch *b, uint32_t iterations, int32_t use_graph, float *gpu_ms)
+{
+    if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    BHIP(hipSetDevice(b->device));
+    if (use_graph && !b->exec) {
+        BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_iteration(b);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->st, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return bfail(MUCA_E_HIP, std::string("graph capture: ") +
+                                         hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        b->graph = g;
+        BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    BHIP(hipEventRecord(b->ev0, b->st));
+    for (uint32_t k = 0; k < iterations; k++) {
+        if (use_graph)
+            BHIP(hipGraphLaunch(b->exec, b->st));
+        else
+            BHIP(launch_iteration(b));
+    }
+    BHIP(hipEventRecord(b->ev1, b->st));
+    BHIP(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (gpu_ms) *gpu_ms = ms;
+    return MUCA_OK;
+}
+
+int muca_batch_status(muca_batch *b, muca_instance_status *out, uint32_t n)
+{
+    if (!b || !out || n != b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    std::vector<muca::BatchInst> cur(n);
+    BHIP(hipMemcpy(cur.data(), b->a.inst, n * sizeof(muca::BatchInst), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
+        const muca::BatchInst &I = cur[i];
+        out[i].iter = I.iter;
+        out[i].nup = I.nup;
+        out[i].width = I.width;
+        out[i].converged = I.converged;
+        out[i].reach_iter = I.reach ? I.reach_iter : 0xFFFFFFFFu;
+        out[i].done_iter = I.converged ? I.done_iter : 0xFFFFFFFFu;
+        out[i].start = I.start;
+        out[i].end = I.end;
+        out[i].dkl = I.dkl;
+        out[i].attempts = I.attempts;
+    }
+    return MUCA_OK;
+}
+
+}  // extern "C"

@@ -21,11 +21,14 @@
 
 namespace muca {
 
+// The walkers wbase .. wbase + blockDim.x - 1 of the launch described by a
+// (every lane of the workgroup calls it together).
 template <int R, bool FP64, int KEYING, bool LDSHIST, bool JOINT>
-__global__ __launch_bounds__(1024) void k_mcs_lane(McsArgs a, uint32_t lds_w)
+__device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint64_t wbase,
+                                         unsigned char *lds)
 {
     typedef typename std::conditional<FP64, double, float>::type T;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    __syncthreads();   // LDS may still be read by a previous call's flush
     T *wt = (T *)lds;
     uint32_t *hist = (uint32_t *)(lds + lds_w);
     const uint32_t N = a.N, L = a.L, Rw = L / 2;
@@ -34,7 +37,7 @@ __global__ __launch_bounds__(1024) void k_mcs_lane(McsArgs a, uint32_t lds_w)
         if (LDSHIST) hist[i] = 0;
     }
     __syncthreads();
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t w = wbase + threadIdx.x;
     if (w < a.W) {
         const uint32_t wid = a.w_offset + (uint32_t)w;
         const uint32_t k1a = KEYING == 0 ? 2u * wid : wid;
@@ -137,6 +140,13 @@ __global__ __launch_bounds__(1024) void k_mcs_lane(McsArgs a, uint32_t lds_w)
     }
 }
 
+template <int R, bool FP64, int KEYING, bool LDSHIST, bool JOINT>
+__global__ __launch_bounds__(1024) void k_mcs_lane(McsArgs a, uint32_t lds_w)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    lane_run<R, FP64, KEYING, LDSHIST, JOINT>(a, lds_w, (uint64_t)blockIdx.x * blockDim.x, lds);
+}
+
 bool lane_eligible(const McsArgs &a, int fp64, size_t lds_max)
 {
     const size_t tsz = fp64 ? 8 : 4;
@@ -181,6 +191,48 @@ hipError_t launch_mcs_lane(hipStream_t st, const Launch &cfg, const McsArgs &a)
                           : launch_lane_r<7, true, 0>(st, cfg, a);
     return cfg.keying ? launch_lane_r<7, false, 1>(st, cfg, a)
                       : launch_lane_r<7, false, 0>(st, cfg, a);
+}
+
+// ---------------------------------------------------------------------------
+// batched engine, lane tiles: a persistent grid takes tiles of blockDim
+// walkers of one instance from a shared counter, largest lattices first.
+// ---------------------------------------------------------------------------
+template <bool FP64>
+__global__ __launch_bounds__(1024) void k_batch_lane(BatchArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_tile = atomicAdd(b.queue, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= b.ntiles) break;
+        const BatchTile tile = b.tiles[t];
+        const BatchInst *I = b.inst + tile.inst;
+        if (I->converged) continue;
+        const McsArgs a = batch_mcs_args(b, *I);
+        const bool hist_lds = I->lane_hist && (uint64_t)blockDim.x * a.nup < (1ull << 32);
+        if (hist_lds)
+            lane_run<7, FP64, 0, true, false>(a, I->lane_lds_w, tile.wbase, lds);
+        else
+            lane_run<7, FP64, 0, false, false>(a, I->lane_lds_w, tile.wbase, lds);
+    }
+}
+
+hipError_t launch_batch_lane(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
+                             uint32_t block, size_t lds_bytes)
+{
+    void (*k)(BatchArgs) = fp64 ? k_batch_lane<true> : k_batch_lane<false>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds_bytes, st, b);
+    return hipGetLastError();
+}
+
+hipError_t prepare_batch_lane(int fp64, size_t lds_bytes)
+{
+    void (*k)(BatchArgs) = fp64 ? k_batch_lane<true> : k_batch_lane<false>;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes);
 }
 
 }  // namespace muca

@@ -71,11 +71,14 @@ __device__ __forceinline__ int32_t row_bcast(int32_t v)
 }  // namespace
 
 // NR: rounds of 16 attempts per half-row, NR * 16 >= L / 2
+// One workgroup's sixteen walkers wbase .. wbase+15 of the launch described
+// by a (all 256 lanes of the workgroup call it together).
 template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
-__global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
+__device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay, uint64_t wbase,
+                                         unsigned char *lds)
 {
     typedef typename std::conditional<FP64, double, float>::type T;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    __syncthreads();   // LDS may still be read by a previous call's flush
     T *wt = (T *)(lds + lay.off_w) + kGuard;   // valid for [-kGuard, N + kGuard]
     uint32_t *hist = (uint32_t *)(lds + lay.off_hist);
     int8_t *latl = (int8_t *)(lds + lay.off_lat);
@@ -87,7 +90,6 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
     TeamRec<T> *rec = (TeamRec<T> *)(tbase + lay.off_rec);      // [Rw]
 
     const uint64_t W = a.W;
-    const uint64_t wbase = (uint64_t)blockIdx.x * kTeams;
     const uint64_t w = wbase + team;
     const bool active = w < W;
 
@@ -310,6 +312,14 @@ __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
     }
 }
 
+template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
+__global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    team_run<R, FP64, KEYING, LATLDS, LDSHIST, JOINT, NR>(a, lay, (uint64_t)blockIdx.x * kTeams,
+                                                         lds);
+}
+
 bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
 {
     if (a.L % 8 != 0 || a.L > 256) return false;
@@ -399,6 +409,67 @@ hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
                           : launch_team_r<7, true, 0>(st, a, lay);
     return cfg.keying ? launch_team_r<7, false, 1>(st, a, lay)
                       : launch_team_r<7, false, 0>(st, a, lay);
+}
+
+// ---------------------------------------------------------------------------
+// batched engine, team tiles: a persistent grid (one workgroup per compute
+// unit; each keeps ~100 KB of LDS) takes tiles of sixteen walkers from a
+// shared counter, largest lattices first, and runs one weight iteration of
+// the tile's instance.  Converged instances are skipped.  Weight keying,
+// Philox4x32-7, walker ids 0..W-1 within each instance, seed = instance seed.
+// ---------------------------------------------------------------------------
+template <bool FP64>
+__global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_tile = atomicAdd(b.queue, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= b.ntiles) break;
+        const BatchTile tile = b.tiles[t];
+        const BatchInst *I = b.inst + tile.inst;
+        if (I->converged) continue;
+        const McsArgs a = batch_mcs_args(b, *I);
+        TeamLayout lay = I->lay;
+        lay.lds_hist = lay.lds_hist && (uint64_t)kTeams * a.nup < (1ull << 32);
+        const uint32_t nr = (lay.row + 15) / 16;
+        const int v = (lay.lat_lds ? 1 : 0) | (lay.lds_hist ? 2 : 0);
+#define MUCA_BT(NR)                                                                         \
+    switch (v) {                                                                            \
+    case 0: team_run<7, FP64, 0, false, false, false, NR>(a, lay, tile.wbase, lds); break;  \
+    case 1: team_run<7, FP64, 0, true, false, false, NR>(a, lay, tile.wbase, lds); break;   \
+    case 2: team_run<7, FP64, 0, false, true, false, NR>(a, lay, tile.wbase, lds); break;   \
+    default: team_run<7, FP64, 0, true, true, false, NR>(a, lay, tile.wbase, lds); break;   \
+    }
+        if (nr <= 1) {
+            MUCA_BT(1)
+        } else if (nr <= 2) {
+            MUCA_BT(2)
+        } else if (nr <= 3) {
+            MUCA_BT(3)
+        } else {
+            MUCA_BT(4)
+        }
+#undef MUCA_BT
+    }
+}
+
+hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
+                             size_t lds_bytes)
+{
+    void (*k)(BatchArgs) = fp64 ? k_batch_team<true> : k_batch_team<false>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kTeams * kTeam), lds_bytes, st, b);
+    return hipGetLastError();
+}
+
+hipError_t prepare_batch_team(int fp64, size_t lds_bytes)
+{
+    void (*k)(BatchArgs) = fp64 ? k_batch_team<true> : k_batch_team<false>;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes);
 }
 
 }  // namespace muca
