@@ -1,21 +1,32 @@
 """Benchmark: flip attempts/ns of the multicanonical weight iteration.
 
-Workload (plain method at reference scale on one GPU): L = 72 (5184 spins,
-5185 D-bins), 4096 walkers, random initial spins (weight keying, seed 1234),
-ln W = 0 initially, recursive weight update, therm = N and NUPDATES = 1 per
-iteration with the reference's x1.01 schedule once the histogram spans all
-bins.  One step = one weight iteration: H2D of ln W, one launch of the update
-kernel (therm + NUPDATES attempts per walker), D2H of the histogram, width /
-D_KL bookkeeping and the O(N) weight update on the host -- the loop body of
-muca_delta_multigpu.cu:374-477.
+Default workload (plain method at reference scale on one GPU): L = 72 (5184
+spins, 5185 D-bins), 4096 walkers, random initial spins (weight keying, seed
+1234), ln W = 0 initially, recursive weight update, therm = N and NUPDATES = 1
+per iteration with the reference's x1.01 schedule once the histogram spans all
+bins.  One step = one weight iteration, the loop body of
+muca_delta_multigpu.cu:374-477: one launch of the update kernel (therm +
+NUPDATES attempts per walker) and the device weight update (hist_range, width,
+D_KL, weight_recursion, normalisation, float copy, schedule), replayed from a
+captured hipGraph with nothing returning to the host between iterations.
 
-Prints one JSON line.  `value` is whole-step throughput (attempts / wall
-time); `roofline` prices the update kernel alone (HIP-event durations on its
-own stream) against HBM at 2 algorithmic bytes per attempt; `cpu_baseline`
-times the CPU oracle (the same chain, threaded over walkers) on a bounded
-sample of the same workload.
+`--workload batch96`: 96 independent instances, L in {32,48,64,72,96,128} x
+Delta in {1.90,1.95,1.98,2.00} x T in {0.5,0.6,0.7,0.8}, seed = instance id,
+`--walkers` walkers each (default 1024), autotuned tiles; one step = one
+weight iteration of every instance.  (The weight problem in D = sum s^2 depends
+on T only; Delta enters at reweighting, so instances that share (L, T) differ
+by seed.)
+
+Prints one JSON line.  `value` is whole-step throughput (attempts / wall time
+of the timed steps); `roofline` prices the update kernel alone (HIP events on
+the batch's stream, over as many further iterations launched eagerly) against
+HBM at 2 algorithmic bytes per attempt; `host_loop` is the same workload
+driven the reference's way (host weight update between launches);
+`cpu_baseline` times the CPU oracle (the same chain, threaded over walkers) on
+a bounded sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--walkers 4096] [--L 72]
+    python bench.py --workload batch96 [--walkers 1024]
     python bench.py --copy-probe BYTES   # plain copy kernel for counter scaling
 """
 import argparse
@@ -31,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 ALGO_BYTES_PER_ATTEMPT = 2.0   # one int8 spin read + one write per site visit
+METRIC = "flip attempts/ns; GPU s to flat H(M) at L=128, surrogate-seeded vs plain"
 
 
 def copy_probe(nbytes):
@@ -45,9 +57,9 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}))
 
 
-def cpu_baseline(L, W, seed, recursion, budget_s, nthreads):
-    """The oracle's weight loop on the host for the same workload: iterations
-    run until budget_s seconds of wall time are spent."""
+def cpu_weight_loop(L, W, seed, recursion, budget_s, nthreads, beta=2.0, max_iters=10000):
+    """The oracle's weight loop on the host: iterations until budget_s seconds
+    of wall time are spent.  Returns (attempts, seconds, iterations, hists)."""
     from oracle import oracle as O
     N = L * L
     lat = O.init_lattice(O.KEYING_WEIGHT, L, W, seed=seed)
@@ -62,7 +74,7 @@ def cpu_baseline(L, W, seed, recursion, budget_s, nthreads):
         if width == N:
             nup = int((nup + 1) * 1.01)
         h = O.mcs(L, lat, E, D, None, keying=0, fp64=False, lnw32=lnw32, iter=k, seed=seed,
-                  therm=N, nupdates=nup, nthreads=nthreads)
+                  therm=N, nupdates=nup, beta=beta, nthreads=nthreads)
         attempts += W * (N + nup)
         hists.append(h)
         s, e = O.hist_range(h)
@@ -74,54 +86,28 @@ def cpu_baseline(L, W, seed, recursion, budget_s, nthreads):
         lnw32 = O.normalise(lnw)
         k += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or k >= 10000:
+        if el >= budget_s or k >= max_iters:
             break
     return attempts, el, k, hists
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--L", type=int, default=72)
-    ap.add_argument("--walkers", type=int, default=4096)
-    ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--trivial", action="store_true", help="trivial instead of recursive update")
-    ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0)
-    args = ap.parse_args()
-
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
-              "build (walkers are independent chains: see DESIGN.md for how it would shard)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import muca_amd as M
-    L, W, seed = args.L, args.walkers, args.seed
+def host_loop(M, L, W, seed, recursion, steps, warmup, kernel):
+    """The reference's arrangement: update launch, D2H histogram, host weight
+    update, H2D weights.  Returns (attempts/ns, mean kernel ms, hists)."""
     N = L * L
-    recursion = not args.trivial
-    ctx = M.Context(L, W, device=0, kernel=args.kernel)
+    ctx = M.Context(L, W, device=0, kernel=kernel)
     ctx.init_lattice(mm=0, gpu=0, seed=seed)
     ctx.compute_state()
-    lnw = np.zeros(N + 1)
-    p_acc = np.zeros(N + 1)
+    lnw, p_acc = np.zeros(N + 1), np.zeros(N + 1)
     ctx.set_log_weight(lnw)
-    state = dict(width=10, nup=1, k=0)
-    gpu_hists = []
+    st = dict(width=10, nup=1, k=0)
+    hists = []
 
     def step():
-        st = state
         if st["width"] == N:
             st["nup"] = int((st["nup"] + 1) * 1.01)
         hist, ms = ctx.run(st["k"], seed, N, st["nup"])
-        gpu_hists.append(hist)
+        hists.append(hist)
         s, e = M.hist_range(hist)
         st["width"] = max(st["width"], (e - s) & 0xFFFFFFFF)
         M.kull_leibler(hist)
@@ -134,57 +120,204 @@ def main():
         st["k"] += 1
         return W * (N + st["nup"]), ms
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
-    attempts, kernel_ms = 0, []
+    att, kms = 0, []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         a, ms = step()
-        attempts += a
-        kernel_ms.append(ms)
+        att += a
+        kms.append(ms)
     wall = time.perf_counter() - t0
-    value = attempts / wall / 1e9
-    avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
-    attempts_per_launch = attempts / args.steps
-    achieved = ALGO_BYTES_PER_ATTEMPT * attempts_per_launch / avg_kernel_s / 1e9
+    ctx.close()
+    return att / wall / 1e9, float(np.mean(kms)), hists
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "update_kernel_L72_W4096_traffic.json")
-    if os.path.exists(tfile) and L == 72 and W == 4096:
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
 
+def batch96(walkers):
+    Ls = [32, 48, 64, 72, 96, 128]
+    deltas = [1.90, 1.95, 1.98, 2.00]
+    Ts = [0.5, 0.6, 0.7, 0.8]
+    ins, meta = [], []
+    for L in Ls:
+        for d in deltas:
+            for T in Ts:
+                i = len(ins)
+                ins.append(dict(L=L, walkers=walkers, seed=i, beta=1.0 / T, recursion=True))
+                meta.append((L, d, T))
+    return ins, meta
+
+
+def run_batch(M, ins, steps, warmup, autotune):
+    """Times `steps` graph-replayed iterations, then as many eager iterations
+    with per-kernel HIP events.  Returns a dict of measurements."""
+    with M.Batch(ins) as b:
+        tune = None
+        if autotune:
+            kind, block, ms = b.autotune(0)
+            tune = {"tiles": {M.KERNEL_TEAM: "team", M.KERNEL_LANE: "lane"}[kind],
+                    "block": block, "iteration_ms": ms}
+        else:
+            b.init(0)
+        if warmup:
+            b.iterate(warmup, use_graph=True)
+        a0 = sum(s["attempts"] for s in b.status())
+        t0 = time.perf_counter()
+        gpu_ms = b.iterate(steps, use_graph=True)   # synchronises before returning
+        wall = time.perf_counter() - t0
+        a1 = sum(s["attempts"] for s in b.status())
+        upd_ms, w_ms = b.iterate_profiled(steps)
+        a2 = sum(s["attempts"] for s in b.status())
+        st = b.status()
+        return dict(attempts=a1 - a0, wall=wall, gpu_ms=gpu_ms, prof_attempts=a2 - a1,
+                    update_ms=upd_ms, weight_ms=w_ms, tiles=b.tiles, block=b.block,
+                    grid=b.grid, ntiles=b.ntiles, tune=tune, status=st)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["c2", "batch96"], default="c2")
+    ap.add_argument("--L", type=int, default=72)
+    ap.add_argument("--walkers", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--trivial", action="store_true", help="trivial instead of recursive update")
+    ap.add_argument("--kernel", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-loop", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
+              "build (walkers are independent chains, so the path would shard by walkers "
+              "with one histogram all-reduce per weight iteration)", file=sys.stderr)
+        sys.exit(2)
+
+    import muca_amd as M
+    nth = min(16, os.cpu_count() or 1)
+    recursion = not args.trivial
+
+    if args.workload == "c2":
+        L, seed = args.L, args.seed
+        W = args.walkers or 4096
+        N = L * L
+        ins = [dict(L=L, walkers=W, seed=seed, beta=2.0, recursion=recursion)]
+        r = run_batch(M, ins, args.steps, args.warmup, autotune=False)
+        workload = f"multicanonical weight iterations, plain method, L={L}, {W} walkers"
+        config = {"workload": workload, "L": L, "walkers": W, "therm": "N",
+                  "nupdates_init": 1, "recursive_update": int(recursion), "seed": seed,
+                  "parallelism": "1 GPU; persistent update kernel over 16-walker tiles, "
+                                 "device weight update, one hipGraph per iteration"}
+    else:
+        W = args.walkers or 1024
+        ins, meta = batch96(W)
+        r = run_batch(M, ins, args.steps, args.warmup, autotune=not args.no_autotune)
+        workload = f"batched weight iterations, 96 instances x {W} walkers"
+        config = {"workload": workload, "instances": 96, "walkers_per_instance": W,
+                  "L": [32, 48, 64, 72, 96, 128], "delta": [1.90, 1.95, 1.98, 2.00],
+                  "T": [0.5, 0.6, 0.7, 0.8], "seed": "instance id", "therm": "N",
+                  "recursive_update": 1,
+                  "parallelism": "1 GPU; one persistent launch over all instances' tiles, "
+                                 "device weight update, one hipGraph per iteration"}
+
+    value = r["attempts"] / r["wall"] / 1e9
+    avg_kernel_s = r["update_ms"] / args.steps * 1e-3
+    att_per_launch = r["prof_attempts"] / args.steps
+    achieved = ALGO_BYTES_PER_ATTEMPT * att_per_launch / avg_kernel_s / 1e9
+
+    # HBM bytes and VALU instructions per launch from the committed rocprofv3
+    # counter passes of this exact workload (none for other shapes)
+    traffic, valu = None, None
+    if args.workload == "c2" and args.L == 72 and W == 4096:
+        tfile = os.path.join(ROOT, "profiles", "update_kernel_L72_W4096_traffic.json")
+        if os.path.exists(tfile):
+            t = json.load(open(tfile))
+            traffic = t.get("bytes_per_launch")
+            insts = t.get("valu_wave_insts_per_launch")
+            if insts:
+                peak = 256 * 2.4e9   # wave64 VALU instructions/s: 256 CUs x 1/clk x 2.4 GHz
+                valu = {"achieved": insts / avg_kernel_s, "peak": peak,
+                        "unit": "wave instructions/s", "frac": insts / avg_kernel_s / peak}
+
+    host = None
     cpu = None
-    if not args.no_cpu_baseline:
-        nth = min(16, os.cpu_count() or 1)
-        att, el, iters, chists = cpu_baseline(L, W, seed, recursion, args.cpu_budget, nth)
-        same = all(np.array_equal(a, b) for a, b in zip(gpu_hists, chists))
+    if args.workload == "c2":
+        if not args.no_host_loop:
+            hv, hk, hhists = host_loop(M, args.L, W, args.seed, recursion, args.steps,
+                                       args.warmup, args.kernel)
+            host = {"value": hv, "kernel_ms_avg": hk,
+                    "note": "same workload, weight update on the host between launches"}
+        if not args.no_cpu_baseline:
+            att, el, iters, chists = cpu_weight_loop(args.L, W, args.seed, recursion,
+                                                     args.cpu_budget, nth)
+            same = None
+            if host is not None:
+                n = min(len(hhists), len(chists))
+                same = all(np.array_equal(a, b) for a, b in zip(hhists[:n], chists[:n]))
+            cpu = {"value": att / el / 1e9, "unit": "attempts/ns", "cores": nth, "kind": "port",
+                   "sample": f"CPU oracle (same chain, {nth} threads over walkers) on the first "
+                             f"{iters} weight iterations of this workload ({att} attempts, "
+                             f"{el:.1f} s); histograms identical to the GPU host-loop run's "
+                             f"for those iterations: {same}"}
+    elif not args.no_cpu_baseline:
+        # a bounded sample: first weight iteration of instances 0, 1, ... (L = 32) until
+        # the budget is spent
+        from oracle import oracle as O
+        att, t0, done = 0, time.perf_counter(), 0
+        for q in ins:
+            L = q["L"]
+            N = L * L
+            lat = O.init_lattice(O.KEYING_WEIGHT, L, q["walkers"], seed=q["seed"])
+            E, D, _ = O.state(L, lat)
+            O.mcs(L, lat, E, D, None, keying=0, fp64=False,
+                  lnw32=np.zeros(N + 1, np.float32), iter=0, seed=q["seed"], therm=N,
+                  nupdates=1, beta=q["beta"], nthreads=nth)
+            att += q["walkers"] * (N + 1)
+            done += 1
+            if time.perf_counter() - t0 >= args.cpu_budget:
+                break
+        el = time.perf_counter() - t0
         cpu = {"value": att / el / 1e9, "unit": "attempts/ns", "cores": nth, "kind": "port",
-               "sample": f"CPU oracle (same chain, {nth} threads over walkers) on the first "
-                         f"{iters} weight iterations of this workload ({att} attempts, "
-                         f"{el:.1f} s); histograms identical to the GPU's: {same}"}
+               "sample": f"CPU oracle (same chain, {nth} threads over walkers): first weight "
+                         f"iteration (NUPDATES = 1) of instances 0..{done - 1}, {att} attempts, "
+                         f"{el:.1f} s"}
 
     out = {
-        "metric": "flip attempts/ns; GPU s to flat H(M) at L=128, surrogate-seeded vs plain",
+        "metric": METRIC,
         "value": value,
         "unit": "attempts/ns",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 spins, u32 Philox, f32 acceptance",
+        "dtype": "int8 spins, u32 Philox, f32 acceptance, f64 weight update",
         "data": "synthetic (random initial spins from the reference's Philox keying)",
-        "config": {"workload": f"multicanonical weight iterations, plain method, L={L}, "
-                               f"{W} walkers", "L": L, "walkers": W, "therm": "N",
-                   "nupdates_init": 1, "recursive_update": int(recursion), "seed": seed,
-                   "parallelism": "1 GPU, one lane per walker"},
+        "config": config,
         "kernel_ms_avg": avg_kernel_s * 1e3,
+        "weight_update_ms_avg": r["weight_ms"] / args.steps,
+        "tiles": {"kind": {M.KERNEL_TEAM: "team", M.KERNEL_LANE: "lane"}[r["tiles"]],
+                  "block": r["block"], "grid": r["grid"], "ntiles": r["ntiles"],
+                  "autotune": r["tune"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "valu_issue": valu,
+        "host_loop": host,
         "cpu_baseline": cpu,
     }
+    if args.workload == "batch96":
+        st = r["status"]
+        out["instances"] = {"converged": int(sum(s["converged"] for s in st)),
+                            "width_eq_N": int(sum(s["reach_iter"] != 0xFFFFFFFF for s in st))}
     print(json.dumps(out))
 
 

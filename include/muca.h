@@ -183,8 +183,9 @@ const char *muca_batch_last_error(void);
 int muca_batch_create(int32_t device, const muca_instance *instances, uint32_t n,
                       int32_t precision, int32_t tiles, muca_batch **batch);
 int muca_batch_destroy(muca_batch *batch);
-int muca_batch_query(const muca_batch *batch, int32_t *tiles, uint32_t *ntiles, uint32_t *grid,
-                     uint64_t *walkers);
+/* tile kind, threads per workgroup, tile count, persistent grid, total walkers */
+int muca_batch_query(const muca_batch *batch, int32_t *tiles, uint32_t *block, uint32_t *ntiles,
+                     uint32_t *grid, uint64_t *walkers);
 /* random lattices (weight keying, jack index mm), E/D/M, ln W = 0, counters reset */
 int muca_batch_init(muca_batch *batch, uint32_t mm);
 int muca_batch_set_log_weight(muca_batch *batch, uint32_t instance, const double *lnw,
@@ -198,6 +199,18 @@ int muca_batch_get_hist(muca_batch *batch, uint32_t instance, uint64_t *hist, ui
  * the elapsed device time */
 int muca_batch_iterate(muca_batch *batch, uint32_t iterations, int32_t use_graph,
                        float *gpu_ms);
+/* the same iterations launched one by one with HIP events around each launch
+ * on the batch's stream: summed durations of the persistent update kernel
+ * and of the device weight update (either pointer may be NULL) */
+int muca_batch_iterate_profiled(muca_batch *batch, uint32_t iterations, float *update_ms,
+                                float *weight_ms);
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * 256 threads per block, muca_delta_multigpu.cu:27): times one iteration of
+ * every eligible tile kind and workgroup size from freshly initialised walkers
+ * (muca_batch_init(mm)), keeps the fastest and leaves the batch initialised
+ * with mm.  kind/block/ms (nullable) report the choice. */
+int muca_batch_autotune(muca_batch *batch, uint32_t mm, int32_t *kind, uint32_t *block,
+                        float *ms);
 int muca_batch_status(muca_batch *batch, muca_instance_status *status, uint32_t n);
 
 /* ---- self-checks usable without a GPU ---- */

@@ -77,8 +77,10 @@ _SIGS = {
                                          ctypes.c_int32, ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_void_p)]),
     "muca_batch_destroy": (ctypes.c_int, [_vp]),
-    "muca_batch_query": (ctypes.c_int, [_vp, _i32p, ctypes.POINTER(_u32),
+    "muca_batch_query": (ctypes.c_int, [_vp, _i32p, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
                                         ctypes.POINTER(_u32), _u64p]),
+    "muca_batch_iterate_profiled": (ctypes.c_int, [_vp, _u32, _f32p, _f32p]),
+    "muca_batch_autotune": (ctypes.c_int, [_vp, _u32, _i32p, ctypes.POINTER(_u32), _f32p]),
     "muca_batch_init": (ctypes.c_int, [_vp, _u32]),
     "muca_batch_set_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
     "muca_batch_get_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
@@ -277,10 +279,14 @@ class Batch:
                                         ctypes.byref(self._h)))
         self.n = len(ins)
         self.N = [q["L"] * q["L"] for q in ins]
-        k, nt, g, w = ctypes.c_int32(), _u32(), _u32(), ctypes.c_uint64()
-        _bcheck(lib().muca_batch_query(self._h, ctypes.byref(k), ctypes.byref(nt),
-                                       ctypes.byref(g), ctypes.byref(w)))
-        self.tiles, self.ntiles, self.grid, self.walkers = k.value, nt.value, g.value, w.value
+        self._query()
+
+    def _query(self):
+        k, bl, nt, g, w = ctypes.c_int32(), _u32(), _u32(), _u32(), ctypes.c_uint64()
+        _bcheck(lib().muca_batch_query(self._h, ctypes.byref(k), ctypes.byref(bl),
+                                       ctypes.byref(nt), ctypes.byref(g), ctypes.byref(w)))
+        self.tiles, self.block, self.ntiles = k.value, bl.value, nt.value
+        self.grid, self.walkers = g.value, w.value
 
     def close(self):
         if self._h:
@@ -322,6 +328,23 @@ class Batch:
         _bcheck(lib().muca_batch_iterate(self._h, iterations, 1 if use_graph else 0,
                                          ctypes.byref(ms)))
         return ms.value
+
+    def iterate_profiled(self, iterations=1):
+        """Eager launches with HIP events around each: returns (summed update
+        kernel ms, summed device weight-update ms)."""
+        u, w = ctypes.c_float(), ctypes.c_float()
+        _bcheck(lib().muca_batch_iterate_profiled(self._h, iterations, ctypes.byref(u),
+                                                  ctypes.byref(w)))
+        return u.value, w.value
+
+    def autotune(self, mm=0):
+        """Pick the fastest tile kind / workgroup size; leaves the batch
+        initialised with mm.  Returns (kind, block, ms of one iteration)."""
+        k, bl, ms = ctypes.c_int32(), _u32(), ctypes.c_float()
+        _bcheck(lib().muca_batch_autotune(self._h, mm, ctypes.byref(k), ctypes.byref(bl),
+                                          ctypes.byref(ms)))
+        self._query()
+        return k.value, bl.value, ms.value
 
     def status(self):
         arr = (InstanceStatus * self.n)()

@@ -34,7 +34,8 @@ hipError_t prepare_batch_lane(int fp64, size_t lds_bytes);
 
 namespace {
 
-constexpr int kUpd = 1024;   // threads of the update workgroup
+constexpr int kUpd = 1024;    // threads of the update workgroup
+constexpr int kChunk = 2048; // recursion steps staged in LDS at a time
 
 template <typename T, typename Op>
 __device__ T block_reduce(T v, T ident, Op op, T *sh)
@@ -76,6 +77,7 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
     __shared__ uint64_t sh_u[16];
     __shared__ double sh_d[16];
     __shared__ int s_conv, s_rec;
+    __shared__ double s_ds[kChunk], s_bn[kChunk], s_x;
     if (blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0;   // tiles of the next iteration
     BatchInst *I = b.inst + blockIdx.x;
     const uint32_t N = I->N;
@@ -146,21 +148,54 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
                     if (c > 1) lnw[i] = lnw[i] - log((double)c);
                 }
             } else {
-                // weight_recursion (:848-858): a sequential prefix over [start, end)
+                // weight_recursion (:848-858).  Each step's increment dS_i and
+                // p_acc update read only the pre-update weights bf and the
+                // histogram, so they are formed in parallel (same expressions,
+                // same order of operations); only lnW[i+1] = lnW[i] + dS_i is a
+                // chain.  Chunks of kChunk steps are staged in LDS and one lane
+                // walks each chunk in order.  A NaN marks "no step" (dS is finite
+                // whenever both counts are nonzero and ln W is finite); a step
+                // that is not taken leaves lnW[i+1] at its old value bf[i+1].
                 double *bf = b.bf + I->w_off, *pacc = b.p_acc + I->w_off;
                 for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) bf[i] = lnw[i];
+                if (threadIdx.x == 0) s_x = lnw[start];
                 __syncthreads();
-                if (threadIdx.x == 0) {
-                    for (int32_t i = start; i < end; i++) {
+                for (int32_t c0 = start; c0 < end; c0 += kChunk) {
+                    const int32_t n = min(kChunk, end - c0);
+                    for (int32_t j = (int32_t)threadIdx.x; j < n; j += blockDim.x) {
+                        const int32_t i = c0 + j;
                         const uint64_t a0 = h[i], a1 = h[i + 1];
+                        double dS = __builtin_nan("");
                         if ((a0 != 0) & (a1 != 0)) {
                             const double p = (double)(a0 * a1) / (double)(a0 + a1);
                             const double k = p / (p + pacc[i]);
-                            const double dS = -bf[i] + bf[i + 1] + k * log((double)a0 / (double)a1);
-                            lnw[i + 1] = lnw[i] + dS;
+                            dS = -bf[i] + bf[i + 1] + k * log((double)a0 / (double)a1);
                             pacc[i] = pacc[i] + p;
                         }
+                        s_ds[j] = dS;
+                        s_bn[j] = bf[i + 1];
                     }
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        double x = s_x;
+                        for (int32_t j = 0; j < n; j += 16) {
+                            double dv[16], bv[16];
+#pragma unroll
+                            for (int u = 0; u < 16; u++) {   // j + u < kChunk: kChunk % 16 == 0
+                                dv[u] = s_ds[j + u];
+                                bv[u] = s_bn[j + u];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 16; u++) {
+                                if (j + u < n) {
+                                    x = (dv[u] == dv[u]) ? x + dv[u] : bv[u];
+                                    lnw[c0 + j + u + 1] = x;
+                                }
+                            }
+                        }
+                        s_x = x;
+                    }
+                    __syncthreads();
                 }
             }
             __syncthreads();
@@ -233,6 +268,8 @@ struct muca_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t lat_bytes = 0, walkers = 0, bins = 0;
+    bool team_ok = false, lane_ok = false;
+    size_t lds_device = 0;
 };
 
 namespace {
@@ -267,6 +304,42 @@ hipError_t launch_iteration(muca_batch *b)
                                                  b->lds_bytes);
     if (e != hipSuccess) return e;
     return muca::launch_batch_update(b->st, b->a);
+}
+
+// Tile list, grid and LDS size for one tile kind; drops any captured graph.
+hipError_t configure(muca_batch *b, int kind, uint32_t block)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->kind = kind;
+    b->block = kind == MUCA_KERNEL_TEAM ? 256 : block;
+    const uint32_t per = kind == MUCA_KERNEL_TEAM ? 16 : b->block;
+    const uint32_t n = (uint32_t)b->inst.size();
+    std::vector<uint32_t> order(n);   // largest lattices first
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return b->inst[x].N > b->inst[y].N; });
+    b->tiles.clear();
+    for (uint32_t i : order)
+        for (uint32_t w = 0; w < b->inst[i].W; w += per) b->tiles.push_back({i, w});
+    size_t need = 0;
+    for (const auto &I : b->inst)
+        need = std::max(need, kind == MUCA_KERNEL_TEAM
+                                  ? I.lay.bytes
+                                  : (size_t)I.lane_lds_w + (I.lane_hist ? (size_t)(I.N + 1) * 4 : 0));
+    b->lds_bytes = need;
+    const uint32_t per_cu_lds = (uint32_t)std::max<size_t>(1, b->lds_device / (need + 256));
+    const uint32_t per_cu_thr = 2048 / b->block;
+    b->grid = b->cus * std::min(per_cu_lds, per_cu_thr);
+    b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->tiles.size());
+    b->a.ntiles = (uint32_t)b->tiles.size();
+    hipError_t e = hipMemcpy(b->d_tiles, b->tiles.data(),
+                             b->tiles.size() * sizeof(muca::BatchTile), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    return kind == MUCA_KERNEL_TEAM ? muca::prepare_batch_team(b->fp64, b->lds_bytes)
+                                    : muca::prepare_batch_lane(b->fp64, b->lds_bytes);
 }
 
 }  // namespace
@@ -336,6 +409,9 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     b->lat_bytes = lat;
     b->walkers = st;
     b->bins = wo;
+    b->team_ok = team_ok;
+    b->lane_ok = lane_ok;
+    b->lds_device = (size_t)lds;
     if (tiles == MUCA_KERNEL_TEAM && !team_ok) {
         batch_free(b);
         return bfail(MUCA_E_ARG, "team tiles need every L % 8 == 0 and L <= 128");
@@ -344,41 +420,24 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
         batch_free(b);
         return bfail(MUCA_E_ARG, "lane tiles need ln W of every instance in LDS");
     }
-    if (tiles == MUCA_KERNEL_TEAM || tiles == MUCA_KERNEL_LANE)
-        b->kind = tiles;
-    else  // same rule as the single-instance contexts: team while walkers are few
-        b->kind = (team_ok && (st <= 128ull * (uint64_t)cu || !lane_ok)) ? MUCA_KERNEL_TEAM
-                                                                          : MUCA_KERNEL_LANE;
-    if (b->kind == MUCA_KERNEL_LANE && !lane_ok) {
+    int kind = tiles;
+    if (tiles != MUCA_KERNEL_TEAM && tiles != MUCA_KERNEL_LANE)
+        // same rule as the single-instance contexts: team while walkers are few
+        kind = (team_ok && (st <= 128ull * (uint64_t)cu || !lane_ok)) ? MUCA_KERNEL_TEAM
+                                                                       : MUCA_KERNEL_LANE;
+    if (kind == MUCA_KERNEL_LANE && !lane_ok) {
         batch_free(b);
         return bfail(MUCA_E_ARG, "no tile kind fits every instance");
     }
-    // tiles, largest lattices first
-    const uint32_t per = b->kind == MUCA_KERNEL_TEAM ? 16 : b->block;
-    std::vector<uint32_t> order(n);
-    for (uint32_t i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t x, uint32_t y) { return b->inst[x].N > b->inst[y].N; });
-    for (uint32_t i : order)
-        for (uint32_t w = 0; w < b->inst[i].W; w += per) b->tiles.push_back({i, w});
-    size_t need = 0;
-    for (const auto &I : b->inst)
-        need = std::max(need, b->kind == MUCA_KERNEL_TEAM
-                                  ? I.lay.bytes
-                                  : (size_t)I.lane_lds_w + (I.lane_hist ? (size_t)(I.N + 1) * 4 : 0));
-    b->lds_bytes = need;
-    const uint32_t per_cu_lds = (uint32_t)std::max<size_t>(1, (size_t)lds / (need + 256));
-    const uint32_t per_cu_thr = b->kind == MUCA_KERNEL_TEAM ? 8 : 2048 / b->block;
-    b->grid = (uint32_t)cu * std::min(per_cu_lds, per_cu_thr);
-    b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->tiles.size());
+    uint64_t max_tiles = 0;   // team tiles (16 walkers) are the most numerous
+    for (const auto &I : b->inst) max_tiles += (I.W + 15) / 16;
     hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     muca::BatchArgs &A = b->a;
-    A.ntiles = (uint32_t)b->tiles.size();
     A.ninst = n;
     if (e == hipSuccess) e = balloc(&A.inst, n);
-    if (e == hipSuccess) e = balloc(&b->d_tiles, b->tiles.size());
+    if (e == hipSuccess) e = balloc(&b->d_tiles, max_tiles);
     if (e == hipSuccess) e = balloc(&A.queue, 1);
     if (e == hipSuccess) e = balloc(&A.lat, lat);
     if (e == hipSuccess) e = balloc(&A.E, st);
@@ -391,12 +450,7 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     if (e == hipSuccess) e = balloc(&A.bf, wo);
     if (e == hipSuccess) e = balloc(&A.p_acc, wo);
     A.tiles = b->d_tiles;
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(muca::BatchTile),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = b->kind == MUCA_KERNEL_TEAM ? muca::prepare_batch_team(b->fp64, b->lds_bytes)
-                                        : muca::prepare_batch_lane(b->fp64, b->lds_bytes);
+    if (e == hipSuccess) e = configure(b, kind, 256);
     if (e != hipSuccess) {
         batch_free(b);
         return bfail(MUCA_E_NOMEM, std::string("batch setup: ") + hipGetErrorString(e));
@@ -411,11 +465,12 @@ int muca_batch_destroy(muca_batch *b)
     return MUCA_OK;
 }
 
-int muca_batch_query(const muca_batch *b, int32_t *kind, uint32_t *tiles, uint32_t *grid,
-                     uint64_t *walkers)
+int muca_batch_query(const muca_batch *b, int32_t *kind, uint32_t *block, uint32_t *tiles,
+                     uint32_t *grid, uint64_t *walkers)
 {
     if (!b) return bfail(MUCA_E_ARG, "NULL batch");
     if (kind) *kind = b->kind;
+    if (block) *block = b->block;
     if (tiles) *tiles = (uint32_t)b->tiles.size();
     if (grid) *grid = b->grid;
     if (walkers) *walkers = b->walkers;
@@ -546,6 +601,77 @@ int muca_batch_iterate(muca_batch *b, uint32_t iterations, int32_t use_graph, fl
     float ms = 0;
     BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if (gpu_ms) *gpu_ms = ms;
+    return MUCA_OK;
+}
+
+int muca_batch_iterate_profiled(muca_batch *b, uint32_t iterations, float *update_ms,
+                                float *weight_ms)
+{
+    if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    BHIP(hipSetDevice(b->device));
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    struct Guard {
+        hipEvent_t *e;
+        ~Guard()
+        {
+            for (int i = 0; i < 3; i++)
+                if (e[i]) hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    for (int i = 0; i < 3; i++) BHIP(hipEventCreate(&ev[i]));
+    double tu = 0, tw = 0;
+    for (uint32_t k = 0; k < iterations; k++) {
+        BHIP(hipEventRecord(ev[0], b->st));
+        BHIP(b->kind == MUCA_KERNEL_TEAM
+                 ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes)
+                 : muca::launch_batch_lane(b->st, b->fp64, b->a, b->grid, b->block, b->lds_bytes));
+        BHIP(hipEventRecord(ev[1], b->st));
+        BHIP(muca::launch_batch_update(b->st, b->a));
+        BHIP(hipEventRecord(ev[2], b->st));
+        BHIP(hipEventSynchronize(ev[2]));
+        float m1 = 0, m2 = 0;
+        BHIP(hipEventElapsedTime(&m1, ev[0], ev[1]));
+        BHIP(hipEventElapsedTime(&m2, ev[1], ev[2]));
+        tu += m1;
+        tw += m2;
+    }
+    if (update_ms) *update_ms = (float)tu;
+    if (weight_ms) *weight_ms = (float)tw;
+    return MUCA_OK;
+}
+
+int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *block, float *ms)
+{
+    if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    BHIP(hipSetDevice(b->device));
+    struct Cand {
+        int kind;
+        uint32_t block;
+    };
+    std::vector<Cand> cands;
+    if (b->team_ok) cands.push_back({MUCA_KERNEL_TEAM, 256});
+    if (b->lane_ok)
+        for (uint32_t blk : {256u, 512u, 1024u}) cands.push_back({MUCA_KERNEL_LANE, blk});
+    float best = 0;
+    Cand pick{b->kind, b->block};
+    for (const Cand &c : cands) {
+        BHIP(configure(b, c.kind, c.block));
+        int rc = muca_batch_init(b, mm);
+        if (rc) return rc;
+        float tu = 0;
+        rc = muca_batch_iterate_profiled(b, 1, &tu, nullptr);
+        if (rc) return rc;
+        if (best == 0 || tu < best) {
+            best = tu;
+            pick = c;
+        }
+    }
+    BHIP(configure(b, pick.kind, pick.block));
+    const int rc = muca_batch_init(b, mm);
+    if (rc) return rc;
+    if (kind) *kind = pick.kind;
+    if (block) *block = b->block;
+    if (ms) *ms = best;
     return MUCA_OK;
 }
 

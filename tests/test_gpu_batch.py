@@ -180,6 +180,25 @@ def test_batch_auto_tiles():
         assert b.walkers == sum(w for (_, w, *_r) in LANE_SET)
 
 
+def test_batch_autotune_leaves_fresh_state():
+    """After the autotuner the batch is as after init(mm): its iterations
+    match a batch created with the chosen tiles and initialised directly."""
+    ins = _instances(TEAM_SET)
+    with M.Batch(ins) as b:
+        kind, block, ms = b.autotune(2)
+        assert kind in (M.KERNEL_TEAM, M.KERNEL_LANE) and ms > 0
+        assert b.tiles == kind and b.block == block
+        b.iterate(3)
+        got = [b.get_log_weight(i) for i in range(b.n)], [b.get_hist(i) for i in range(b.n)]
+    with M.Batch(ins, tiles=kind) as b:
+        b.init(2)
+        u, w = b.iterate_profiled(3)
+        assert u > 0 and w > 0
+        for i in range(b.n):
+            assert np.array_equal(b.get_log_weight(i), got[0][i])
+            assert np.array_equal(b.get_hist(i), got[1][i])
+
+
 def test_batch_rejects_bad_instances():
     with pytest.raises(M.MucaError):
         M.Batch([dict(L=7, walkers=4, seed=1)])
