@@ -25,8 +25,17 @@ driven the reference's way (host weight update between launches);
 `cpu_baseline` times the CPU oracle (the same chain, threaded over walkers) on
 a bounded sample of the same workload.
 
+`--workload seeded`: GPU seconds to a flat histogram (width == N and
+D_KL < 1e-4, muca_delta_multigpu.cu:441-447) for an unseen (L, T, Delta) =
+(`--L`, 0.55, 1.97), 4096 walkers, trivial update.  "seeded": ln W from the
+surrogate (trained on the GPU on the committed corpus of converged weights,
+L <= 40), NUPDATES from the start = 16 N^2 (`--nup-factor`; the
+reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the reference's defaults).  Each
+run stops at its device-time cap; a run that hits it reports a lower bound.
+
     python bench.py [--steps K] [--warmup W] [--walkers 4096] [--L 72]
     python bench.py --workload batch96 [--walkers 1024]
+    python bench.py --workload seeded [--L 128] [--mode both|seeded|plain]
     python bench.py --copy-probe BYTES   # plain copy kernel for counter scaling
 """
 import argparse
@@ -178,8 +187,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "batch96"], default="c2")
-    ap.add_argument("--L", type=int, default=72)
+    ap.add_argument("--workload", choices=["c2", "batch96", "seeded"], default="c2")
+    ap.add_argument("--mode", choices=["both", "seeded", "plain"], default="both")
+    ap.add_argument("--T", type=float, default=0.55)
+    ap.add_argument("--delta", type=float, default=1.97)
+    ap.add_argument("--cap-seeded", type=float, default=600.0)
+    ap.add_argument("--cap-plain", type=float, default=600.0)
+    ap.add_argument("--nup-factor", type=float, default=16.0,
+                    help="seeded run: NUPDATES from the start = factor * N^2")
+    ap.add_argument("--L", type=int, default=None, help="72 for c2, 128 for seeded")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--trivial", action="store_true", help="trivial instead of recursive update")
@@ -203,7 +219,12 @@ def main():
     import muca_amd as M
     nth = min(16, os.cpu_count() or 1)
     recursion = not args.trivial
+    if args.workload == "seeded":
+        seeded_workload(M, args)
+        return
 
+    if args.L is None:
+        args.L = 72
     if args.workload == "c2":
         L, seed = args.L, args.seed
         W = args.walkers or 4096
@@ -318,6 +339,85 @@ def main():
         st = r["status"]
         out["instances"] = {"converged": int(sum(s["converged"] for s in st)),
                             "width_eq_N": int(sum(s["reach_iter"] != 0xFFFFFFFF for s in st))}
+    print(json.dumps(out))
+
+
+def seeded_workload(M, args):
+    from muca_amd import surrogate as S
+    L = args.L or 128
+    W = args.walkers or 4096
+    beta = 1.0 / args.T
+    N = L * L
+
+    def log(tag):
+        def f(gpu, st):
+            s = st[0]
+            print(f"{tag}: gpu {gpu:.1f}s iter {s['iter']} width {s['width']} D_KL "
+                  f"{s['dkl']:.2e} NUPDATES {s['nup']}", file=sys.stderr, flush=True)
+        return f
+
+    res = {}
+    train_info = None
+    if args.mode in ("both", "seeded"):
+        X, t, w = S.corpus(S.load_corpus())
+        net, train_info = S.train(X, t, w)
+        lnw0 = S.seed_log_weight(net, L, beta, args.delta)
+        net.close()
+        nup0 = S.equilibrium_nupdates(L, args.nup_factor)
+        print(f"seeded: trained on {len(X)} samples in {train_info['gpu_s']:.2f} GPU s, "
+              f"NUPDATES from the start {nup0}", file=sys.stderr, flush=True)
+        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta, recursion=False,
+                                  init_nup=nup0)], lnw_init=[lnw0], stratified=False,
+                            max_gpu_s=args.cap_seeded, chunk=1, log=log("seeded"))
+        res["seeded"] = out[0]
+    if args.mode in ("both", "plain"):
+        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta, recursion=False,
+                                  init_nup=1)], stratified=False, max_gpu_s=args.cap_plain,
+                            chunk=50, log=log("plain"))
+        res["plain"] = out[0]
+
+    def summary(o):
+        st = o["status"]
+        return {"flat": bool(st["converged"]), "gpu_s": o["gpu_s"], "iterations": st["iter"],
+                "width_reached_N_at": None if st["reach_iter"] == 0xFFFFFFFF else st["reach_iter"],
+                "last_D_KL": st["dkl"], "attempts": st["attempts"]}
+
+    sm = {k: summary(v) for k, v in res.items()}
+    ratio = None
+    if "seeded" in sm and "plain" in sm and sm["seeded"]["flat"]:
+        r = sm["plain"]["gpu_s"] / sm["seeded"]["gpu_s"]
+        ratio = {"value": r, "lower_bound": not sm["plain"]["flat"]}
+    agree = None
+    if "seeded" in res and "plain" in res and sm["seeded"]["flat"] and sm["plain"]["flat"]:
+        d = res["seeded"]["lnw"] - res["plain"]["lnw"]
+        agree = {"max_abs_lnw_diff": float(np.max(np.abs(d))),
+                 "rms_lnw_diff": float(np.sqrt(np.mean(d * d)))}
+    main_run = sm.get("seeded") or sm.get("plain")
+    out = {
+        "metric": METRIC,
+        "value": main_run["gpu_s"],
+        "unit": "GPU s to flat H",
+        "n_gpus": 1,
+        "steps": main_run["iterations"],
+        "warmup": 0,
+        "ms_per_step": main_run["gpu_s"] * 1e3 / max(1, main_run["iterations"]),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 spins, f32 acceptance, f64 weight update; surrogate bf16 MFMA / f32",
+        "data": "synthetic (random initial spins from the reference's Philox keying)",
+        "config": {"workload": f"weight iteration to flat H, L={L}, T={args.T}, "
+                               f"Delta={args.delta}, {W} walkers, trivial update",
+                   "seeded_nupdates": f"{args.nup_factor} N^2",
+                   "L": L, "T": args.T, "delta": args.delta, "walkers": W,
+                   "criterion": "width == N and D_KL < 1e-4",
+                   "caps_gpu_s": {"seeded": args.cap_seeded, "plain": args.cap_plain}},
+        "runs": sm,
+        "speedup_plain_over_seeded": ratio,
+        "lnw_agreement": agree,
+        "surrogate_training": None if train_info is None else
+        {k: v for k, v in train_info.items() if k != "loss_curve"},
+    }
     print(json.dumps(out))
 
 

@@ -82,6 +82,12 @@ int muca_query(const muca_ctx *ctx, muca_info *info);
 /* Initial random lattice (muca_delta_multigpu.cu:340-351 for mm = jack index;
  * produce2_multigpu.cu:126-138 ignores mm), generated on the device. */
 int muca_init_lattice(muca_ctx *ctx, uint32_t mm, uint32_t gpu_index, uint32_t seed);
+/* Stratified start (no counterpart in the reference; for runs seeded with
+ * good weights, whose walkers would otherwise spend ~N^2 attempts diffusing
+ * out of the random start's D ~ 2N/3): the same Philox stream, but walker w
+ * of W has a nonzero spin with probability (w + 1/2)/W, sign - for
+ * u < p/2, + for p/2 <= u < p, so the walkers start spread over D. */
+int muca_init_lattice_stratified(muca_ctx *ctx, uint32_t mm, uint32_t gpu_index, uint32_t seed);
 /* Lattice in the reference's layout s[site * walkers + walker]
  * (muca_delta_multigpu.cu:151,800). */
 int muca_set_lattice(muca_ctx *ctx, const int8_t *site_major, uint64_t bytes);
@@ -188,6 +194,8 @@ int muca_batch_query(const muca_batch *batch, int32_t *tiles, uint32_t *block, u
                      uint32_t *grid, uint64_t *walkers);
 /* random lattices (weight keying, jack index mm), E/D/M, ln W = 0, counters reset */
 int muca_batch_init(muca_batch *batch, uint32_t mm);
+/* the same with the stratified start of muca_init_lattice_stratified */
+int muca_batch_init_stratified(muca_batch *batch, uint32_t mm);
 int muca_batch_set_log_weight(muca_batch *batch, uint32_t instance, const double *lnw,
                               uint32_t nbins);
 int muca_batch_get_log_weight(muca_batch *batch, uint32_t instance, double *lnw,
@@ -211,7 +219,40 @@ int muca_batch_iterate_profiled(muca_batch *batch, uint32_t iterations, float *u
  * with mm.  kind/block/ms (nullable) report the choice. */
 int muca_batch_autotune(muca_batch *batch, uint32_t mm, int32_t *kind, uint32_t *block,
                         float *ms);
+/* (the autotuner re-initialises with the start of the last init call) */
 int muca_batch_status(muca_batch *batch, muca_instance_status *status, uint32_t n);
+
+/* ---- weight surrogate (no counterpart in the reference) ----
+ * An MLP 4 -> 128 -> 128 -> 128 -> 1 (SiLU) predicting ln W(D)/N from four
+ * features of (D/N, 1/L, beta, Delta); its output seeds the init-weight file
+ * that multigpu_weight's `continuing` mode reads (muca_delta_multigpu.cu:223-235)
+ * in place of the extrapolated initial weights.  FP32 master parameters; the
+ * 128 x 128 products run on MFMA with BF16 operands and FP32 accumulation; the
+ * input and output layers use BF16-rounded weights in FP32.  Parameters are
+ * packed as W1[128][4], b1[128], W2[128][128], b2, W3[128][128], b3, w4[128],
+ * b4 (W[o][k]: output o, input k). */
+typedef struct muca_mlp muca_mlp;
+const char *muca_mlp_last_error(void);
+uint32_t muca_mlp_param_count(void);
+int muca_mlp_create(int32_t device, muca_mlp **mlp);
+int muca_mlp_destroy(muca_mlp *mlp);
+/* also resets the Adam moments and step count */
+int muca_mlp_set_params(muca_mlp *mlp, const float *params, uint32_t n);
+int muca_mlp_get_params(muca_mlp *mlp, float *params, uint32_t n);
+/* y[i] for inputs x[i][4] */
+int muca_mlp_predict(muca_mlp *mlp, const float *x, uint32_t n, float *y);
+/* gradient of the mean squared error over the n samples (no update);
+ * grad (nullable) receives muca_mlp_param_count() values, loss (nullable) the MSE */
+int muca_mlp_gradient(muca_mlp *mlp, const float *x, const float *t, uint32_t n, float *grad,
+                      float *loss);
+/* resident training set (the caller shuffles it); w (nullable) weights each
+ * sample's squared error (loss = sum w (y - t)^2 / batch) */
+int muca_mlp_set_data(muca_mlp *mlp, const float *x, const float *t, const float *w, uint32_t n);
+/* `steps` Adam steps on consecutive minibatches of `batch` samples (a multiple
+ * of 64), learning rate cosine-annealed from lr to lr_final; loss of the last
+ * minibatch and the elapsed device time (both nullable) */
+int muca_mlp_train(muca_mlp *mlp, uint32_t steps, uint32_t batch, float lr, float lr_final,
+                   float *loss, float *gpu_ms);
 
 /* ---- self-checks usable without a GPU ---- */
 /* the acceptance log evaluated by the host instantiation of the device code */

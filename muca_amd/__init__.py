@@ -82,11 +82,24 @@ _SIGS = {
     "muca_batch_iterate_profiled": (ctypes.c_int, [_vp, _u32, _f32p, _f32p]),
     "muca_batch_autotune": (ctypes.c_int, [_vp, _u32, _i32p, ctypes.POINTER(_u32), _f32p]),
     "muca_batch_init": (ctypes.c_int, [_vp, _u32]),
+    "muca_batch_init_stratified": (ctypes.c_int, [_vp, _u32]),
+    "muca_init_lattice_stratified": (ctypes.c_int, [_vp, _u32, _u32, _u32]),
     "muca_batch_set_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
     "muca_batch_get_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
     "muca_batch_get_hist": (ctypes.c_int, [_vp, _u32, _u64p, _u32]),
     "muca_batch_iterate": (ctypes.c_int, [_vp, _u32, ctypes.c_int32, _f32p]),
     "muca_batch_status": (ctypes.c_int, [_vp, ctypes.POINTER(InstanceStatus), _u32]),
+    "muca_mlp_last_error": (ctypes.c_char_p, []),
+    "muca_mlp_param_count": (ctypes.c_uint32, []),
+    "muca_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "muca_mlp_destroy": (ctypes.c_int, [_vp]),
+    "muca_mlp_set_params": (ctypes.c_int, [_vp, _f32p, _u32]),
+    "muca_mlp_get_params": (ctypes.c_int, [_vp, _f32p, _u32]),
+    "muca_mlp_predict": (ctypes.c_int, [_vp, _f32p, _u32, _f32p]),
+    "muca_mlp_gradient": (ctypes.c_int, [_vp, _f32p, _f32p, _u32, _f32p, _f32p]),
+    "muca_mlp_set_data": (ctypes.c_int, [_vp, _f32p, _f32p, _f32p, _u32]),
+    "muca_mlp_train": (ctypes.c_int, [_vp, _u32, _u32, ctypes.c_float, ctypes.c_float, _f32p,
+                                      _f32p]),
     "muca_last_error": (ctypes.c_char_p, []),
     "muca_default_walkers": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32)]),
     "muca_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
@@ -194,8 +207,9 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def init_lattice(self, mm=0, gpu=0, seed=SEED):
-        _check(lib().muca_init_lattice(self._h, mm, gpu, seed))
+    def init_lattice(self, mm=0, gpu=0, seed=SEED, stratified=False):
+        fn = lib().muca_init_lattice_stratified if stratified else lib().muca_init_lattice
+        _check(fn(self._h, mm, gpu, seed))
 
     def set_lattice(self, site_major):
         a = np.ascontiguousarray(site_major, dtype=np.int8)
@@ -305,8 +319,9 @@ class Batch:
     def __exit__(self, *a):
         self.close()
 
-    def init(self, mm=0):
-        _bcheck(lib().muca_batch_init(self._h, mm))
+    def init(self, mm=0, stratified=False):
+        fn = lib().muca_batch_init_stratified if stratified else lib().muca_batch_init
+        _bcheck(fn(self._h, mm))
 
     def set_log_weight(self, i, lnw):
         a = np.ascontiguousarray(lnw, np.float64)
@@ -350,6 +365,75 @@ class Batch:
         arr = (InstanceStatus * self.n)()
         _bcheck(lib().muca_batch_status(self._h, arr, self.n))
         return [{f: getattr(s, f) for f, _ in InstanceStatus._fields_} for s in arr]
+
+
+def _mcheck(rc):
+    if rc != 0:
+        raise MucaError(f"libmuca mlp error {rc}: {lib().muca_mlp_last_error().decode()}")
+
+
+class Mlp:
+    """The weight surrogate network on one device (include/muca.h, muca_mlp_*)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _mcheck(lib().muca_mlp_create(device, ctypes.byref(self._h)))
+        self.nparams = lib().muca_mlp_param_count()
+
+    def close(self):
+        if self._h:
+            lib().muca_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        _mcheck(lib().muca_mlp_set_params(self._h, _p(a, _f32p), a.size))
+
+    def get_params(self):
+        a = np.empty(self.nparams, np.float32)
+        _mcheck(lib().muca_mlp_get_params(self._h, _p(a, _f32p), a.size))
+        return a
+
+    def predict(self, x):
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        y = np.empty(len(xa), np.float32)
+        _mcheck(lib().muca_mlp_predict(self._h, _p(xa, _f32p), len(xa), _p(y, _f32p)))
+        return y
+
+    def gradient(self, x, t):
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        ta = np.ascontiguousarray(t, np.float32)
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        _mcheck(lib().muca_mlp_gradient(self._h, _p(xa, _f32p), _p(ta, _f32p), len(xa),
+                                        _p(g, _f32p), ctypes.byref(loss)))
+        return g, loss.value
+
+    def set_data(self, x, t, w=None):
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        ta = np.ascontiguousarray(t, np.float32)
+        wa = None if w is None else np.ascontiguousarray(w, np.float32)
+        _mcheck(lib().muca_mlp_set_data(self._h, _p(xa, _f32p), _p(ta, _f32p), _p(wa, _f32p),
+                                        len(xa)))
+
+    def train(self, steps, batch, lr, lr_final):
+        """Returns (last minibatch loss, device ms)."""
+        loss, ms = ctypes.c_float(), ctypes.c_float()
+        _mcheck(lib().muca_mlp_train(self._h, steps, batch, lr, lr_final, ctypes.byref(loss),
+                                     ctypes.byref(ms)))
+        return loss.value, ms.value
 
 
 # ---- host pieces of the weight iteration (muca_delta_multigpu.cu:431-476) ----

@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
 LIB_SOURCES = ["muca_kernels.hip", "muca_team.hip", "muca_lane.hip", "muca_batch.hip",
-               "muca_api.cpp"]
+               "muca_mlp.hip", "muca_api.cpp"]
 CLI_SOURCES = {"multigpu_weight": "cli_weight.cpp", "produce_gpu": "cli_produce.cpp"}
 
 

@@ -37,9 +37,10 @@ struct Launch {
     size_t lds_max;  // bytes of LDS available per block
 };
 
+// strat_W = 0: the reference's uniform start; > 0: stratified over strat_W walkers
 hipError_t launch_init_lattice(hipStream_t st, int8_t *lat, uint64_t W, uint32_t N,
                                int keying, int rounds, uint32_t mm, uint32_t gpu,
-                               uint32_t seed, uint32_t w_offset);
+                               uint32_t seed, uint32_t w_offset, uint32_t strat_W = 0);
 hipError_t launch_state(hipStream_t st, const int8_t *lat, uint64_t W, uint32_t N,
                         const uint32_t *naver, int32_t *E, int32_t *D, int32_t *M);
 hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a);

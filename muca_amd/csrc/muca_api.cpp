@@ -277,6 +277,16 @@ int muca_init_lattice(muca_ctx *c, uint32_t mm, uint32_t gpu_index, uint32_t see
     return MUCA_OK;
 }
 
+int muca_init_lattice_stratified(muca_ctx *c, uint32_t mm, uint32_t gpu_index, uint32_t seed)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(muca::launch_init_lattice(c->st, c->lat, c->W, c->N, c->p.keying, (int)c->p.rounds,
+                                      mm, gpu_index, seed, 0, (uint32_t)c->W));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return MUCA_OK;
+}
+
 int muca_set_lattice(muca_ctx *c, const int8_t *src, uint64_t bytes)
 {
     if (!c || !src) return fail(MUCA_E_ARG, "NULL argument");

@@ -270,6 +270,7 @@ struct muca_batch {
     uint64_t lat_bytes = 0, walkers = 0, bins = 0;
     bool team_ok = false, lane_ok = false;
     size_t lds_device = 0;
+    int stratified = 0;   // start of the last init
 };
 
 namespace {
@@ -305,6 +306,8 @@ hipError_t launch_iteration(muca_batch *b)
     if (e != hipSuccess) return e;
     return muca::launch_batch_update(b->st, b->a);
 }
+
+int batch_init(muca_batch *b, uint32_t mm, int stratified);
 
 // Tile list, grid and LDS size for one tile kind; drops any captured graph.
 hipError_t configure(muca_batch *b, int kind, uint32_t block)
@@ -479,7 +482,21 @@ int muca_batch_query(const muca_batch *b, int32_t *kind, uint32_t *block, uint32
 
 int muca_batch_init(muca_batch *b, uint32_t mm)
 {
+    return batch_init(b, mm, 0);
+}
+
+int muca_batch_init_stratified(muca_batch *b, uint32_t mm)
+{
+    return batch_init(b, mm, 1);
+}
+
+}  // extern "C"
+
+namespace {
+int batch_init(muca_batch *b, uint32_t mm, int stratified)
+{
     if (!b) return bfail(MUCA_E_ARG, "NULL batch");
+    b->stratified = stratified;
     BHIP(hipSetDevice(b->device));
     muca::BatchArgs &A = b->a;
     uint32_t maxN = 0;
@@ -497,7 +514,7 @@ int muca_batch_init(muca_batch *b, uint32_t mm)
     for (size_t i = 0; i < b->inst.size(); i++) {
         muca::BatchInst &I = b->inst[i];
         BHIP(muca::launch_init_lattice(b->st, A.lat + I.lat_off, I.W, I.N, MUCA_KEYING_WEIGHT, 7,
-                                       mm, 0, I.seed, 0));
+                                       mm, 0, I.seed, 0, stratified ? I.W : 0));
         if (I.L != cur_L) {   // neighbour table of this L (muca_delta_multigpu.cu:157-172)
             BHIP(hipStreamSynchronize(b->st));   // the previous table may still be read
             naver.assign(4ull * I.N, 0);
@@ -532,6 +549,9 @@ int muca_batch_init(muca_batch *b, uint32_t mm)
     BHIP(hipMemset(A.p_acc, 0, b->bins * 8));
     return MUCA_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int muca_batch_set_log_weight(muca_batch *b, uint32_t i, const double *lnw, uint32_t nbins)
 {
@@ -656,7 +676,7 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
     Cand pick{b->kind, b->block};
     for (const Cand &c : cands) {
         BHIP(configure(b, c.kind, c.block));
-        int rc = muca_batch_init(b, mm);
+        int rc = batch_init(b, mm, b->stratified);
         if (rc) return rc;
         float tu = 0;
         rc = muca_batch_iterate_profiled(b, 1, &tu, nullptr);
@@ -667,7 +687,7 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
         }
     }
     BHIP(configure(b, pick.kind, pick.block));
-    const int rc = muca_batch_init(b, mm);
+    const int rc = batch_init(b, mm, b->stratified);
     if (rc) return rc;
     if (kind) *kind = pick.kind;
     if (block) *block = b->block;

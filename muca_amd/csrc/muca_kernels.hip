@@ -20,10 +20,15 @@ namespace muca {
 // initial lattice (muca_delta_multigpu.cu:340-351; produce2_multigpu.cu:126-138)
 // one thread per (walker, block of four sites); stores are coalesced over walkers
 // ---------------------------------------------------------------------------
+// strat_W > 0 selects the stratified start (no counterpart in the reference):
+// walker wid of strat_W gets a nonzero spin with probability
+// p = (wid + 1/2) / strat_W, sign - for u < p/2 and + for p/2 <= u < p, from
+// the same Philox stream, so that the walkers start spread over D = 0..N.
 template <int R>
 __global__ __launch_bounds__(256) void k_init_lattice(int8_t *lat, uint64_t W, uint32_t N,
                                                       int keying, uint32_t mm, uint32_t gpu,
-                                                      uint32_t seed, uint32_t w_offset)
+                                                      uint32_t seed, uint32_t w_offset,
+                                                      uint32_t strat_W)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nblk = (N + 3) / 4;
@@ -37,28 +42,31 @@ __global__ __launch_bounds__(256) void k_init_lattice(int8_t *lat, uint64_t W, u
                           10u * wid + mm * 2u, 0xdecafbadu);
     else
         r = philox4x32<R>(gpu * 1000u, seed, blk + 1u, 0xBADCAFE3u, wid * 10u, 0xdecafbadu);
+    const double p = strat_W ? ((double)wid + 0.5) / (double)strat_W : 0.0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t site = 4u * blk + (uint32_t)j;
         if (site < N) {
             const double u = (double)((float)(1u | (r.v[j] >> 8)) * 0x1p-24f);
-            lat[(uint64_t)site * W + w] = (int8_t)((u < 1 / 3.0) + (u < 2 / 3.0) - 1);
+            const int8_t sv = strat_W ? (int8_t)(u < p ? (u < 0.5 * p ? -1 : 1) : 0)
+                                      : (int8_t)((u < 1 / 3.0) + (u < 2 / 3.0) - 1);
+            lat[(uint64_t)site * W + w] = sv;
         }
     }
 }
 
 hipError_t launch_init_lattice(hipStream_t st, int8_t *lat, uint64_t W, uint32_t N,
                                int keying, int rounds, uint32_t mm, uint32_t gpu,
-                               uint32_t seed, uint32_t w_offset)
+                               uint32_t seed, uint32_t w_offset, uint32_t strat_W)
 {
     const uint64_t total = W * ((N + 3) / 4);
     const dim3 grid((unsigned)((total + 255) / 256));
     if (rounds == 10)
         hipLaunchKernelGGL(k_init_lattice<10>, grid, dim3(256), 0, st, lat, W, N, keying, mm,
-                           gpu, seed, w_offset);
+                           gpu, seed, w_offset, strat_W);
     else
         hipLaunchKernelGGL(k_init_lattice<7>, grid, dim3(256), 0, st, lat, W, N, keying, mm,
-                           gpu, seed, w_offset);
+                           gpu, seed, w_offset, strat_W);
     return hipGetLastError();
 }
 

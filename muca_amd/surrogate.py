@@ -1,0 +1,214 @@
+"""Surrogate seeding of the multicanonical weights.
+
+The reference seeds hard weight runs from an extrapolated initial weight file
+(`continuing` mode, muca_delta_multigpu.cu:223-235, `initweight_L128_T050.txt`:
+N+1 whitespace-separated values of ln W).  This module replaces the
+extrapolation by a small network trained on weights that the batched engine
+itself has converged:
+
+    corpus  = converged ln W(D) of many (L, T) instances      (muca_amd.Batch)
+    net     = MLP 4 -> 128 -> 128 -> 128 -> 1, SiLU            (muca_amd.Mlp, MFMA kernels)
+    seed    = ln W for an unseen (L, T, Delta), written as the init-weight file
+
+Representation.  ln W(D) is extensive: ln W(D)/N -> f(D/N; beta) + O(1/L).  Its
+dominant part is the ground-state line: the best D-spin configuration has
+E_J = -2D, so ln W(D) ~ -2 beta D - S(D).  The network predicts the residual
+r = 10 (ln W/N + 2 beta m), m = D/N, which is small and smooth, from the
+features (m, 16/L, 2 (beta - 1.5), 10 (Delta - 1.95)).  Delta does not enter
+the weight problem in D (it enters at reweighting), so the network learns to
+ignore it; it is kept as an input so one model serves the whole (T, Delta)
+grid.  The seed never touches the acceptance arithmetic: it only replaces the
+initial ln W, and the weight iteration with its flatness test stays the
+judge.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import Batch, Mlp, KERNEL_AUTO, REF32
+
+RESIDUAL_SCALE = 10.0
+DELTAS = (1.90, 1.95, 1.98, 2.00)
+CORPUS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                           "weight_corpus.npz")
+
+
+def features(m, L, beta, delta):
+    m = np.asarray(m, np.float64)
+    x = np.empty((m.size, 4), np.float32)
+    x[:, 0] = m
+    x[:, 1] = 16.0 / L
+    x[:, 2] = 2.0 * (beta - 1.5)
+    x[:, 3] = 10.0 * (delta - 1.95)
+    return x
+
+
+def targets(lnw, L, beta):
+    N = L * L
+    m = np.arange(N + 1) / N
+    return (RESIDUAL_SCALE * (np.asarray(lnw, np.float64) / N + 2.0 * beta * m)).astype(np.float32)
+
+
+def lnw_from_output(y, L, beta):
+    """ln W(D), normalised to ln W(0) = 0, from network outputs at m = D/N."""
+    N = L * L
+    m = np.arange(N + 1) / N
+    lnw = N * (np.asarray(y, np.float64) / RESIDUAL_SCALE - 2.0 * beta * m)
+    return lnw - lnw[0]
+
+
+def corpus(runs):
+    """runs: iterable of dicts with L, beta, delta, lnw (converged, N+1 values).
+    Returns features X [n][4], targets t [n] and sample weights w [n]: an
+    error e in the target is an error N e / 10 in ln W, so samples are
+    weighted by N (normalised to mean 1)."""
+    xs, ts, ws = [], [], []
+    for r in runs:
+        L = r["L"]
+        N = L * L
+        xs.append(features(np.arange(N + 1) / N, L, r["beta"], r["delta"]))
+        ts.append(targets(r["lnw"], L, r["beta"]))
+        ws.append(np.full(N + 1, float(N)))
+    w = np.concatenate(ws)
+    return np.concatenate(xs), np.concatenate(ts), (w / w.mean()).astype(np.float32)
+
+
+def init_params(seed=0):
+    """Uniform(-1/sqrt(fan_in), +1/sqrt(fan_in)) weights (input layer +-1/2),
+    zero biases, in the packed layout of include/muca.h."""
+    H = 128
+    rng = np.random.default_rng(seed)
+    parts = [rng.uniform(-0.5, 0.5, H * 4), np.zeros(H),
+             rng.uniform(-1, 1, H * H) / np.sqrt(H), np.zeros(H),
+             rng.uniform(-1, 1, H * H) / np.sqrt(H), np.zeros(H),
+             rng.uniform(-1, 1, H) / np.sqrt(H), np.zeros(1)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def train(X, t, w=None, *, device=0, steps=20000, batch=4096, lr=2e-3, lr_final=1e-5, seed=0):
+    """Trains a fresh network on the GPU.  Samples are shuffled once per epoch
+    on the host and made resident; returns (net, info)."""
+    rng = np.random.default_rng(seed)
+    n = len(X)
+    batch = min(batch, (n // 64) * 64)
+    net = Mlp(device)
+    net.set_params(init_params(seed))
+    nb = n // batch
+    per_epoch = max(1, nb)
+    done, gpu_ms, losses = 0, 0.0, []
+    # one resident shuffle per epoch; the cosine schedule spans all steps, so the
+    # device trains `steps` steps in chunks of one epoch with the matching
+    # slice of the schedule
+    while done < steps:
+        perm = rng.permutation(n)
+        net.set_data(X[perm], t[perm], None if w is None else w[perm])
+        k = min(per_epoch, steps - done)
+        a = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * done / steps))
+        b = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * (done + k) / steps))
+        loss, ms = net.train(k, batch, float(a), float(b))
+        gpu_ms += ms
+        losses.append(loss)
+        done += k
+    _, full = net.gradient(X, t)
+    return net, dict(steps=steps, batch=batch, samples=n, gpu_s=gpu_ms / 1e3,
+                     final_loss=full, loss_curve=losses)
+
+
+def generate_corpus(path, Ls=(16, 24, 32, 40, 48), Ts=(0.5, 0.6, 0.7, 0.8), walkers=16384,
+                    max_gpu_s=300.0, log=None):
+    """Converges one batched instance per (L, T) with the reference's weight
+    loop (trivial update, therm = N, NUPDATES = 1 rising x1.01 once the
+    histogram spans all bins; stratified start; seed 100 + instance index)
+    and stores the converged ln W as L{L}_T{T} in an .npz file.  Instances
+    not flat within max_gpu_s of device time are left out.  The committed
+    corpus (muca_amd/data/weight_corpus.npz) was made with these defaults."""
+    ins, meta = [], []
+    for L in Ls:
+        for T in Ts:
+            ins.append(dict(L=L, walkers=walkers, seed=100 + len(ins), beta=1.0 / T,
+                            recursion=False))
+            meta.append((L, T))
+    out, gpu = converge(ins, stratified=True, max_gpu_s=max_gpu_s, log=log)
+    keep = {f"L{L}_T{T}": o["lnw"] for (L, T), o in zip(meta, out) if o["status"]["converged"]}
+    np.savez_compressed(path, **keep)
+    return keep, gpu
+
+
+def load_corpus(path=CORPUS_PATH, exclude=()):
+    """Runs for corpus(): every stored (L, T) once per Delta of the grid."""
+    d = np.load(path)
+    runs = []
+    for k in sorted(d.files):
+        if k in exclude:
+            continue
+        L = int(k.split("_")[0][1:])
+        T = float(k.split("_")[1][1:])
+        for delta in DELTAS:
+            runs.append(dict(L=L, beta=1.0 / T, delta=delta, lnw=d[k]))
+    return runs
+
+
+def seed_log_weight(net, L, beta, delta):
+    N = L * L
+    y = net.predict(features(np.arange(N + 1) / N, L, beta, delta))
+    return lnw_from_output(y, L, beta)
+
+
+def write_init_weight(path, lnw):
+    """The init-weight file read by multigpu_weight's continuing mode
+    (muca_delta_multigpu.cu:223-235): N+1 whitespace-separated values."""
+    with open(path, "w") as f:
+        for v in lnw:
+            f.write(repr(float(v)) + "\n")
+
+
+def equilibrium_nupdates(L, factor=4):
+    """NUPDATES of the seeded protocol: factor * N^2 attempts per walker.  A
+    flat-histogram walker needs ~N^2 attempts to cross D = 0..N, so with a
+    factor of a few one iteration's histogram reflects the current weights and
+    the trivial update corrects a seed's errors in a few iterations instead of
+    the ~100-fold NUPDATES ramp of a run from ln W = 0.  Larger lattices need a
+    larger factor for D_KL < 1e-4 (statistics per bin)."""
+    N = L * L
+    return int(factor * N * N)
+
+
+def converge(instances, *, lnw_init=None, stratified=True, max_gpu_s=60.0, chunk=20,
+             device=0, tiles=KERNEL_AUTO, log=None):
+    """Runs the batched weight iteration until every instance meets the
+    reference's flatness test (width == N and D_KL < 1e-4) or max_gpu_s of
+    device time is spent.  Returns per-instance dicts with the converged (or
+    last) ln W, the status, and the device seconds spent while it was active."""
+    b = Batch(instances, device=device, precision=REF32, tiles=tiles)
+    try:
+        b.init(0, stratified=stratified)
+        if lnw_init is not None:
+            for i, lw in enumerate(lnw_init):
+                if lw is not None:
+                    b.set_log_weight(i, lw)
+        n = len(instances)
+        gpu = 0.0
+        active_s = [0.0] * n
+        t0 = time.time()
+        while True:
+            st0 = b.status()
+            ms = b.iterate(chunk)
+            st = b.status()
+            gpu += ms / 1e3
+            # device time is charged to the instances that were active in the chunk
+            act = [i for i in range(n) if not st0[i]["converged"]]
+            for i in act:
+                active_s[i] += ms / 1e3
+            if log and (time.time() - t0) > 10:
+                t0 = time.time()
+                log(gpu, st)
+            if all(s["converged"] for s in st) or gpu >= max_gpu_s:
+                break
+        out = []
+        for i in range(n):
+            out.append(dict(status=st[i], lnw=b.get_log_weight(i), hist=b.get_hist(i),
+                            gpu_s=active_s[i]))
+        return out, gpu
+    finally:
+        b.close()

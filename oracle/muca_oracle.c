@@ -154,8 +154,17 @@ void orc_tables(uint32_t L, uint32_t *naver, uint32_t *order)
 }
 
 /* R5  initial lattice (muca_delta_multigpu.cu:340-351; produce2_multigpu.cu:126-138) */
+void orc_init_walker_strat(int keying, uint32_t L, uint32_t w, uint32_t mm, uint32_t gpu,
+                           uint32_t seed, unsigned rounds, uint32_t strat_W, int8_t *s);
+
 void orc_init_walker(int keying, uint32_t L, uint32_t w, uint32_t mm, uint32_t gpu,
                      uint32_t seed, unsigned rounds, int8_t *s)
+{
+    orc_init_walker_strat(keying, L, w, mm, gpu, seed, rounds, 0, s);
+}
+
+void orc_init_walker_strat(int keying, uint32_t L, uint32_t w, uint32_t mm, uint32_t gpu,
+                           uint32_t seed, unsigned rounds, uint32_t strat_W, int8_t *s)
 {
     uint32_t N = L * L;
     uint32_t key[2], c[4], r[4] = {0, 0, 0, 0};
@@ -183,7 +192,12 @@ void orc_init_walker(int keying, uint32_t L, uint32_t w, uint32_t mm, uint32_t g
             orc_philox4x32(rounds, c, key, r);
         }
         float u = orc_u01fixedpt_f32(r[i % 4]);
-        s[i] = (int8_t)(((double)u < 1 / 3.0) + ((double)u < 2 / 3.0) - 1);
+        if (strat_W) {
+            const double p = ((double)w + 0.5) / (double)strat_W;
+            s[i] = (int8_t)((double)u < p ? ((double)u < 0.5 * p ? -1 : 1) : 0);
+        } else {
+            s[i] = (int8_t)(((double)u < 1 / 3.0) + ((double)u < 2 / 3.0) - 1);
+        }
     }
 }
 

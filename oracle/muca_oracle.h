@@ -63,6 +63,11 @@ void orc_tables(uint32_t L, uint32_t *naver, uint32_t *order);
  * spin = (u < 1/3.0) + (u < 2/3.0) - 1 with u = u01fixedpt<float>. */
 void orc_init_walker(int keying, uint32_t L, uint32_t walker, uint32_t mm, uint32_t gpu,
                      uint32_t seed, unsigned rounds, int8_t *s);
+/* The engine's stratified start (no counterpart in the reference): the same
+ * stream, spin = (u < p) ? (u < p/2 ? -1 : +1) : 0 with p = (walker + 1/2) /
+ * strat_W; strat_W = 0 is orc_init_walker. */
+void orc_init_walker_strat(int keying, uint32_t L, uint32_t walker, uint32_t mm, uint32_t gpu,
+                           uint32_t seed, unsigned rounds, uint32_t strat_W, int8_t *s);
 
 /* ---- R7: per-walker E_J, D, M (muca_delta_multigpu.cu:69-83,548-571;
  * produce2_multigpu.cu:66-89). */

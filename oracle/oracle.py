@@ -62,6 +62,9 @@ def lib():
         L.orc_init_walker.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_uint, _u8p]
+        L.orc_init_walker_strat.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint, ctypes.c_uint32, _u8p]
         L.orc_state.argtypes = [ctypes.c_uint32, _u32p, _u8p, _i32p, _i32p, _i32p]
         L.orc_mcs_batch.argtypes = [ctypes.POINTER(RunParams), _f32p, _f64p, ctypes.c_uint32,
                                     ctypes.c_uint32, _u8p, _i32p, _i32p, _i32p, _u64p, _u64p,
@@ -124,13 +127,16 @@ def tables(L):
     return naver, order
 
 
-def init_lattice(keying, L, walkers, mm=0, gpu=0, seed=SEED_DEFAULT, rounds=7, w0=0):
-    """Walker-major lattices [walker][site] for walkers w0..w0+walkers-1."""
+def init_lattice(keying, L, walkers, mm=0, gpu=0, seed=SEED_DEFAULT, rounds=7, w0=0,
+                 strat_W=0):
+    """Walker-major lattices [walker][site] for walkers w0..w0+walkers-1
+    (strat_W > 0: the stratified start over strat_W walkers)."""
     N = L * L
     lat = np.zeros((walkers, N), dtype=np.int8)
     for j in range(walkers):
         row = lat[j]
-        lib().orc_init_walker(keying, L, w0 + j, mm, gpu, seed, rounds, _p(row, _u8p))
+        lib().orc_init_walker_strat(keying, L, w0 + j, mm, gpu, seed, rounds, strat_W,
+                                    _p(row, _u8p))
     return lat
 
 

@@ -31,11 +31,12 @@ def _instances(spec):
 class OracleInstance:
     """One instance's chain on the oracle, fed the device's weights."""
 
-    def __init__(self, orc, q):
+    def __init__(self, orc, q, stratified=False):
         self.orc, self.q = orc, q
         self.L, self.N = q["L"], q["L"] * q["L"]
         self.lat = orc.init_lattice(M.KEYING_WEIGHT, self.L, q["walkers"], mm=0, gpu=0,
-                                    seed=q["seed"], rounds=7)
+                                    seed=q["seed"], rounds=7,
+                                    strat_W=q["walkers"] if stratified else 0)
         self.E, self.D, self.Mg = orc.state(self.L, self.lat)
         self.width, self.nup = 10, q["init_nup"]
         self.p_acc = np.zeros(self.N + 1)
@@ -67,12 +68,12 @@ class OracleInstance:
         return hist, lnw, s, e, dkl, nup, conv
 
 
-def _run_against_oracle(orc, spec, tiles, use_graph, iters):
+def _run_against_oracle(orc, spec, tiles, use_graph, iters, stratified=False):
     ins = _instances(spec)
     with M.Batch(ins, tiles=tiles) as b:
         assert b.tiles == tiles
-        b.init(0)
-        ref = [OracleInstance(orc, q) for q in ins]
+        b.init(0, stratified=stratified)
+        ref = [OracleInstance(orc, q, stratified) for q in ins]
         for k in range(iters):
             before = [b.get_log_weight(i) for i in range(b.n)]
             b.iterate(1, use_graph=use_graph)
@@ -108,6 +109,10 @@ def test_batch_team_tiles_match_oracle(orc, use_graph):
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_lane_tiles_match_oracle(orc, use_graph):
     _run_against_oracle(orc, LANE_SET, M.KERNEL_LANE, use_graph, iters=4)
+
+
+def test_batch_stratified_start_matches_oracle(orc):
+    _run_against_oracle(orc, TEAM_SET, M.KERNEL_TEAM, True, iters=3, stratified=True)
 
 
 def test_batch_fp64_lane_first_iteration(orc):

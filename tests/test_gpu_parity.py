@@ -64,6 +64,16 @@ def test_init_lattice_and_state(orc, keying, L, W, mm, gpu, seed, rounds):
     assert np.array_equal(E, rE) and np.array_equal(D, rD) and np.array_equal(Mg, rM)
 
 
+@pytest.mark.parametrize("keying", [M.KEYING_WEIGHT, M.KEYING_PRODUCE])
+def test_stratified_start_matches_oracle(orc, keying):
+    L, W = 12, 77
+    with M.Context(L, W, keying=keying) as ctx:
+        ctx.init_lattice(mm=1, gpu=0, seed=31, stratified=True)
+        lat = ctx.get_lattice()
+    ref = orc.init_lattice(keying, L, W, mm=1, gpu=0, seed=31, strat_W=W)
+    assert np.array_equal(lat, ref.T)
+
+
 CASES = [
     # L, W, precision, keying, beta, rounds, therm, nup
     (16, 16, M.REF32, M.KEYING_WEIGHT, 2.0, 7, 256, 1),

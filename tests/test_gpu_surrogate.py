@@ -1,0 +1,95 @@
+"""What the trained surrogate gains over the method without it, at a size a
+test can afford: a corpus of converged weights at L = 8, 12, 16 (T = 0.5 ..
+0.8) from the batched engine, the network trained on it with the MFMA
+kernels, then an unseen (L, T, Delta) = (24, 0.55, 1.97).
+
+Three runs of the same weight loop and flatness test (width == N, D_KL <
+1e-4), 4096 walkers, trivial update, reference start:
+  plain     ln W = 0, NUPDATES = 1 (the reference's defaults);
+  schedule  ln W = 0, NUPDATES = 4 N^2 from the start (no network);
+  seeded    surrogate ln W, NUPDATES = 4 N^2.
+The seeded run must reach a flat histogram, faster than both others, with
+weights that agree with the plain run's."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import muca_amd as M
+from muca_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+L_T, T_T, D_T, W = 24, 0.55, 1.97, 4096
+
+
+@pytest.fixture(scope="module")
+def seed_lnw():
+    ins, meta = [], []
+    for L in (8, 12, 16):
+        for T in (0.5, 0.6, 0.7, 0.8):
+            ins.append(dict(L=L, walkers=W, seed=100 + len(ins), beta=1 / T, recursion=False))
+            meta.append((L, T))
+    out, gpu = S.converge(ins, stratified=True, max_gpu_s=40.0)
+    runs = []
+    for (L, T), o in zip(meta, out):
+        if o["status"]["converged"]:
+            for d in S.DELTAS:
+                runs.append(dict(L=L, beta=1 / T, delta=d, lnw=o["lnw"]))
+    print(f"corpus: {len(runs) // 4} of {len(ins)} instances flat in {gpu:.1f} GPU s")
+    assert len(runs) >= 4 * 9
+    X, t, w = S.corpus(runs)
+    net, info = S.train(X, t, w, steps=6000)
+    print(f"training: {info['samples']} samples, loss {info['final_loss']:.3g}, "
+          f"{info['gpu_s']:.2f} GPU s")
+    lnw = S.seed_log_weight(net, L_T, 1 / T_T, D_T)
+    net.close()
+    assert lnw[0] == 0.0 and np.all(np.isfinite(lnw))
+    return lnw
+
+
+def _run(lnw0, nup0, cap, chunk):
+    out, _ = S.converge([dict(L=L_T, walkers=W, seed=777, beta=1 / T_T, recursion=False,
+                              init_nup=nup0)], lnw_init=None if lnw0 is None else [lnw0],
+                        stratified=False, max_gpu_s=cap, chunk=chunk)
+    return out[0]
+
+
+def test_seeded_run_beats_plain_and_schedule(seed_lnw):
+    nup0 = S.equilibrium_nupdates(L_T)
+    seeded = _run(seed_lnw, nup0, 20.0, 1)
+    plain = _run(None, 1, 40.0, 50)
+    sched = _run(None, nup0, max(2.0 * seeded["gpu_s"], 1.0), 1)
+    for name, r in (("seeded", seeded), ("plain", plain), ("schedule", sched)):
+        st = r["status"]
+        print(f"{name}: flat {st['converged']} GPU s {r['gpu_s']:.2f} iterations {st['iter']} "
+              f"D_KL {st['dkl']:.2e}")
+    assert seeded["status"]["converged"]
+    assert plain["status"]["converged"]
+    assert seeded["gpu_s"] * 1.5 < plain["gpu_s"]
+    # the network, not the schedule, makes the difference
+    assert not sched["status"]["converged"] or seeded["gpu_s"] * 1.5 < sched["gpu_s"]
+    d = seeded["lnw"] - plain["lnw"]
+    print(f"ln W seeded - plain: max {np.max(np.abs(d)):.3f} rms {np.sqrt(np.mean(d * d)):.3f}")
+    assert np.sqrt(np.mean(d * d)) < 0.5
+
+
+def test_seed_file_drives_the_drop_in_cli(seed_lnw, tmp_path):
+    """The seed written as the init-weight file is read by multigpu_weight's
+    continuing mode (muca_delta_multigpu.cu:223-235)."""
+    path = tmp_path / "initweight.txt"
+    S.write_init_weight(path, seed_lnw)
+    back = np.loadtxt(path)
+    assert np.array_equal(back, seed_lnw)
+    exe = f"{M.BIN_DIR}/multigpu_weight"
+    p = subprocess.run([exe, str(L_T), "0", "0", "1", str(S.equilibrium_nupdates(L_T)), "0", "1",
+                        f"--init-weight={path}", f"--outdir={tmp_path}", "--walkers=4096",
+                        "--max-iter=3", f"--beta={1 / T_T}"], capture_output=True, text=True,
+                       timeout=100)
+    assert p.returncode == 0, p.stderr
+    fss = tmp_path / "fss" / f"L{L_T}"
+    short = next(f for f in fss.iterdir() if f.name.startswith("shortiter"))
+    rows = [ln.split() for ln in short.read_text().splitlines()[1:]]
+    assert len(rows) == 3
+    # with the seed, the first iteration's histogram already spans most of D
+    assert int(rows[0][1]) >= L_T * L_T // 2

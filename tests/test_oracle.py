@@ -150,6 +150,30 @@ def test_init_lattice_is_uniform(orc):
     assert np.all(np.abs(frac - 1 / 3) < 0.02)
 
 
+def test_stratified_start(orc):
+    """strat_W = 0 is the reference start; otherwise walker w's spins follow
+    the closed form of the same uniforms, so D/N tracks (w + 1/2)/W and the
+    signs are balanced."""
+    L, W = 16, 32
+    a = orc.init_lattice(0, L, 4, seed=9, w0=3)
+    b = orc.init_lattice(0, L, 4, seed=9, w0=3, strat_W=0)
+    assert np.array_equal(a, b)
+    lat = orc.init_lattice(0, L, W, seed=9, strat_W=W)
+    ref = orc.init_lattice(0, L, W, seed=9)          # same stream: recover u's bins
+    N = L * L
+    D = (lat != 0).sum(1) / N
+    p = (np.arange(W) + 0.5) / W
+    assert np.all(np.abs(D - p) < 4 * np.sqrt(p * (1 - p) / N) + 1e-9)
+    # the reference start has -1 exactly where u >= 2/3, and +1 where u < 1/3:
+    # there the stratified spin is 0 when p < 2/3, and -1 when p/2 > 1/3
+    for w in range(W):
+        if p[w] < 2 / 3:
+            assert np.all(lat[w][ref[w] == -1] == 0)
+        if p[w] / 2 > 1 / 3:
+            assert np.all(lat[w][ref[w] == 1] == -1)
+    assert abs((lat == 1).sum() - (lat == -1).sum()) < 0.05 * (lat != 0).sum()
+
+
 # ---- R7 state ----------------------------------------------------------------
 def test_state_brute_force(orc):
     L = 12
