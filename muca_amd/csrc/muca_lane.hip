@@ -107,8 +107,11 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
                             m += ds;
                         }
                         if (phase) {
-                            if (LDSHIST)
-                                atomicAdd(&hist[d], 1u);
+                            if (LDSHIST) {
+                                // a u32 bin that wraps carries 2^32 into HBM
+                                if (atomicAdd(&hist[d], 1u) == 0xFFFFFFFFu)
+                                    atomicAdd((unsigned long long *)&a.hist[d], 1ull << 32);
+                            }
                             else
                                 atomicAdd((unsigned long long *)&a.hist[d], 1ull);
                             if (JOINT) {
@@ -160,7 +163,7 @@ static hipError_t launch_lane_r(hipStream_t st, const Launch &cfg, const McsArgs
     const size_t tsz = FP64 ? 8 : 4;
     const uint32_t lds_w = (uint32_t)(((a.N + 1) * tsz + 15) & ~(size_t)15);
     const size_t lds_h = (size_t)(a.N + 1) * 4;
-    const bool hist_lds = lds_w + lds_h <= cfg.lds_max && (uint64_t)block * a.nup < (1ull << 32);
+    const bool hist_lds = lds_w + lds_h <= cfg.lds_max;
     const size_t bytes = lds_w + (hist_lds ? lds_h : 0);
     const bool joint = a.joint != nullptr;
     void (*k)(McsArgs, uint32_t);
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(1024) void k_batch_lane(BatchArgs b)
         const BatchInst *I = b.inst + tile.inst;
         if (I->converged) continue;
         const McsArgs a = batch_mcs_args(b, *I);
-        const bool hist_lds = I->lane_hist && (uint64_t)blockDim.x * a.nup < (1ull << 32);
+        const bool hist_lds = I->lane_hist;
         if (hist_lds)
             lane_run<7, FP64, 0, true, false>(a, I->lane_lds_w, tile.wbase, lds);
         else

@@ -175,6 +175,9 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t dE = -ds * nb;
                     const int32_t ddel = new_s * new_s - old_s * old_s;
                     const T fde = (T)(-(double)dE * a.beta);
+                    // the correctly rounded acceptance log (a 16-wide mask needs it
+                    // for every attempt; screening each of the 16 comparisons with
+                    // the hardware log costs more than it saves here)
                     const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
                     const T tlog = (T)logu;
                     uint32_t mk = 0;
@@ -185,9 +188,13 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         diff = diff - win[k + 1];
                         mk = mk + mk + (tlog < diff ? 1u : 0u);
                     }
-                    mask[r] = mk;
-                    info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
-                                    : ((8 << 2) | (2 << 7));
+                    // bits 0..15: accept at D = dlo + k; bits 16..17: dDel
+                    mask[r] = mk | (valid ? ((uint32_t)ddel & 3u) << 16 : 0u);
+                    if (JOINT)
+                        info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
+                                        : ((8 << 2) | (2 << 7));
+                    else
+                        info[r] = valid ? ((dE + 8) << 2) | ((ds + 2) << 7) : ((8 << 2) | (2 << 7));
                     if (valid) {
                         TeamRec<T> rc;
                         rc.fde = fde;
@@ -206,13 +213,13 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     dv[r] = 0;
                     mv[r] = 0;
                 }
+                const int32_t d_start = d;
                 auto step = [&](auto lc, int r) {
                     constexpr int l = decltype(lc)::value;
                     const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
-                    const int32_t ij = row_bcast<l>(info[r]);
-                    const int32_t ddel = __builtin_amdgcn_sbfe(ij, 0, 2);
+                    const int32_t ddel = __builtin_amdgcn_sbfe((int32_t)mj, 16, 2);
                     const uint32_t off = (uint32_t)(d - dlo);
-                    uint32_t acc = (mj >> (off & 31u)) & 1u;
+                    uint32_t acc = (mj >> (off & 15u)) & 1u;
                     const bool oob = off >= (uint32_t)kWin;
                     if (__builtin_amdgcn_ballot_w64(oob)) {   // rare: D left the window
                         const TeamRec<T> rc = rec[16 * r + l];
@@ -223,8 +230,9 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     }
                     d += (int32_t)acc * ddel;
                     accb[r] |= acc << l;
-                    dv[r] = tl == l ? d : dv[r];
                     if (JOINT) {
+                        const int32_t ij = row_bcast<l>(info[r]);
+                        dv[r] = tl == l ? d : dv[r];
                         m += (int32_t)acc * (((ij >> 7) & 7) - 2);
                         mv[r] = tl == l ? m : mv[r];
                     }
@@ -251,6 +259,24 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 }
 #undef MUCA_STEP_IF
 #undef MUCA_STEP
+                if (!JOINT) {
+                    // D after each lane's own attempt: the chain's start plus the
+                    // dDel of the accepted attempts up to it, counted from the
+                    // team's 16-bit slices of two ballots
+                    const uint32_t sh = (uint32_t)(tid & 48);
+                    const uint32_t upto = (2u << tl) - 1u;
+                    int32_t drun = d_start;
+#pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        const int32_t my = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
+                        const uint32_t up = (uint32_t)(__builtin_amdgcn_ballot_w64(my > 0) >> sh) &
+                                            0xffffu & accb[r];
+                        const uint32_t dn = (uint32_t)(__builtin_amdgcn_ballot_w64(my < 0) >> sh) &
+                                            0xffffu & accb[r];
+                        dv[r] = drun + __builtin_popcount(up & upto) - __builtin_popcount(dn & upto);
+                        drun += __builtin_popcount(up) - __builtin_popcount(dn);
+                    }
+                }
                 // ---- C: accepted spins, dE / dM sums, histogram
                 int32_t de = 0, dm = 0;
 #pragma unroll
@@ -265,8 +291,11 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                             dm += ds;
                         }
                         if (phase) {
-                            if (LDSHIST)
-                                atomicAdd(&hist[dv[r]], 1u);
+                            if (LDSHIST) {
+                                // a u32 bin that wraps carries 2^32 into HBM
+                                if (atomicAdd(&hist[dv[r]], 1u) == 0xFFFFFFFFu)
+                                    atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull << 32);
+                            }
                             else
                                 atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull);
                             if (JOINT) {
@@ -339,8 +368,7 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     lay.off_w = (uint32_t)off;
     off += al((a.N + 1 + 2 * kGuard) * tsz);
     // histogram and lattice in LDS when they fit (histogram first)
-    const bool hist_fits = off + al((a.N + 1) * 4) <= lds_max &&
-                           (uint64_t)kTeams * a.nup < (1ull << 32);
+    const bool hist_fits = off + al((a.N + 1) * 4) <= lds_max;
     lay.lds_hist = hist_fits;
     lay.off_hist = (uint32_t)off;
     if (hist_fits) off += al((a.N + 1) * 4);
@@ -433,8 +461,7 @@ __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
         const BatchInst *I = b.inst + tile.inst;
         if (I->converged) continue;
         const McsArgs a = batch_mcs_args(b, *I);
-        TeamLayout lay = I->lay;
-        lay.lds_hist = lay.lds_hist && (uint64_t)kTeams * a.nup < (1ull << 32);
+        const TeamLayout lay = I->lay;
         const uint32_t nr = (lay.row + 15) / 16;
         const int v = (lay.lat_lds ? 1 : 0) | (lay.lds_hist ? 2 : 0);
 #define MUCA_BT(NR)                                                                         \
