@@ -12,10 +12,10 @@ captured hipGraph with nothing returning to the host between iterations.
 
 `--workload batch96`: 96 independent instances, L in {32,48,64,72,96,128} x
 Delta in {1.90,1.95,1.98,2.00} x T in {0.5,0.6,0.7,0.8}, seed = instance id,
-`--walkers` walkers each (default 1024), autotuned tiles; one step = one
-weight iteration of every instance.  (The weight problem in D = sum s^2 depends
-on T only; Delta enters at reweighting, so instances that share (L, T) differ
-by seed.)
+`--walkers` walkers each (default 4096, as the L = 72 workload), autotuned
+tiles; one step = one weight iteration of every instance.  (The weight
+problem in D = sum s^2 depends on T only; Delta enters at reweighting, so
+instances that share (L, T) differ by seed.)
 
 Prints one JSON line.  `value` is whole-step throughput (attempts / wall time
 of the timed steps); `roofline` prices the update kernel alone (HIP events on
@@ -30,12 +30,22 @@ D_KL < 1e-4, muca_delta_multigpu.cu:441-447) for an unseen (L, T, Delta) =
 (`--L`, 0.55, 1.97), 4096 walkers, trivial update.  "seeded": ln W from the
 surrogate (trained on the GPU on the committed corpus of converged weights,
 L <= 40), NUPDATES from the start = 16 N^2 (`--nup-factor`; the
-reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the reference's defaults).  Each
-run stops at its device-time cap; a run that hits it reports a lower bound.
+reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the
+reference's defaults).  Each run stops at its device-time cap; a run that
+hits it reports a lower bound.
+
+`--workload c5`: large production, L = 256 (65 536 spins), 16 384 walkers,
+T = 0.5, produce keying and FP64 acceptance, ln W from the surrogate, one
+launch of therm = N and NUPDATES = N per walker into ten per-block joint
+histograms H(D, M) (ten x 17.2 GB in HBM), then FP64 reweighting on the
+device to <|M|>, chi and the Binder cumulant over a Delta grid with
+jackknife errors; block 0 at Delta = 1.98 is checked against the CPU
+oracle's long-double observables().
 
     python bench.py [--steps K] [--warmup W] [--walkers 4096] [--L 72]
-    python bench.py --workload batch96 [--walkers 1024]
+    python bench.py --workload batch96 [--walkers 4096]
     python bench.py --workload seeded [--L 128] [--mode both|seeded|plain]
+    python bench.py --workload c5 [--L 256] [--walkers 16384]
     python bench.py --copy-probe BYTES   # plain copy kernel for counter scaling
 """
 import argparse
@@ -187,7 +197,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "batch96", "seeded"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "batch96", "seeded", "c5"], default="c2")
+    ap.add_argument("--therm-mult", type=int, default=1, help="c5: therm = mult * N")
+    ap.add_argument("--nup-mult", type=int, default=1, help="c5: NUPDATES = mult * N")
+    ap.add_argument("--blocks", type=int, default=10, help="c5: jackknife blocks")
     ap.add_argument("--mode", choices=["both", "seeded", "plain"], default="both")
     ap.add_argument("--T", type=float, default=0.55)
     ap.add_argument("--delta", type=float, default=1.97)
@@ -222,6 +235,9 @@ def main():
     if args.workload == "seeded":
         seeded_workload(M, args)
         return
+    if args.workload == "c5":
+        c5_workload(M, args)
+        return
 
     if args.L is None:
         args.L = 72
@@ -237,7 +253,7 @@ def main():
                   "parallelism": "1 GPU; persistent update kernel over 16-walker tiles, "
                                  "device weight update, one hipGraph per iteration"}
     else:
-        W = args.walkers or 1024
+        W = args.walkers or 4096
         ins, meta = batch96(W)
         r = run_batch(M, ins, args.steps, args.warmup, autotune=not args.no_autotune)
         workload = f"batched weight iterations, 96 instances x {W} walkers"
@@ -417,6 +433,70 @@ def seeded_workload(M, args):
         "lnw_agreement": agree,
         "surrogate_training": None if train_info is None else
         {k: v for k, v in train_info.items() if k != "loss_curve"},
+    }
+    print(json.dumps(out))
+
+
+def c5_workload(M, args):
+    from muca_amd import surrogate as S
+    from oracle import oracle as O
+    L = args.L or 256
+    W = args.walkers or 16384
+    N = L * L
+    beta, delta0, seed = 2.0, 1.98, args.seed
+    B = args.blocks
+    X, t, w = S.corpus(S.load_corpus())
+    net, _ = S.train(X, t, w)
+    lnw = S.seed_log_weight(net, L, beta, delta0)
+    net.close()
+    therm, nup = args.therm_mult * N, args.nup_mult * N
+    print(f"c5: L={L} walkers={W} blocks={B} therm={therm} NUPDATES={nup}", file=sys.stderr,
+          flush=True)
+    with M.Context(L, W, keying=M.KEYING_PRODUCE, precision=M.FP64, joint_blocks=B) as ctx:
+        ctx.init_lattice(mm=0, gpu=0, seed=seed)
+        ctx.compute_state()
+        ctx.set_log_weight(lnw)
+        _, ms = ctx.run(0, seed, therm, nup)
+        print(f"c5: production launch {ms:.1f} ms ({ctx.info.kernel=})", file=sys.stderr,
+              flush=True)
+        deltas = np.round(np.linspace(1.90, 2.00, 11), 4)
+        t0 = time.perf_counter()
+        absM, chi, binder = ctx.reweight(deltas)
+        rw_ms = (time.perf_counter() - t0) * 1e3
+        joint0 = ctx.get_joint(0)
+    i0 = int(np.argmin(np.abs(deltas - delta0)))
+    wsh = lnw.copy()
+    O.weight_shift(beta, delta0, wsh)
+    a_ref, c_ref = O.observables(beta, N, joint0, O.boltz(beta, delta0, wsh))
+    del joint0
+    obs = []
+    for i, d in enumerate(deltas):
+        am, ae = M.jackknife(absM[i] / N)
+        cm, ce = M.jackknife(chi[i])
+        bm, be = M.jackknife(binder[i])
+        obs.append({"delta": float(d), "abs_m": [am, ae], "chi": [cm, ce], "binder": [bm, be]})
+    attempts = W * (therm + nup)
+    out = {
+        "metric": METRIC,
+        "value": attempts / (ms * 1e-3) / 1e9,
+        "unit": "attempts/ns",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8 spins, f64 acceptance, u64 joint histograms, f64 reweighting",
+        "data": "synthetic (random initial spins from the reference's Philox keying)",
+        "config": {"workload": f"large production, L={L}, {W} walkers, T=0.5, "
+                               f"{B} jackknife blocks", "L": L, "walkers": W,
+                   "therm": f"{args.therm_mult} N", "nupdates": f"{args.nup_mult} N",
+                   "joint_bins_per_block": (N + 1) * (N + 2) // 2,
+                   "weights": "surrogate seed (not converged)"},
+        "kernel": {1: "generic", 2: "team", 3: "lane"}.get(ctx.info.kernel, "?"),
+        "reweight_ms": rw_ms,
+        "observables": obs,
+        "oracle_block0": {"delta": delta0, "abs_M_device": float(absM[i0, 0]),
+                          "abs_M_oracle": a_ref, "chi_device": float(chi[i0, 0]),
+                          "chi_oracle": c_ref,
+                          "rel_diff_abs_M": abs(absM[i0, 0] - a_ref) / abs(a_ref),
+                          "rel_diff_chi": abs(chi[i0, 0] - c_ref) / abs(c_ref)},
     }
     print(json.dumps(out))
 

@@ -18,8 +18,8 @@ features (m, 16/L, 2 (beta - 1.5), 10 (Delta - 1.95)).  Delta does not enter
 the weight problem in D (it enters at reweighting), so the network learns to
 ignore it; it is kept as an input so one model serves the whole (T, Delta)
 grid.  The seed never touches the acceptance arithmetic: it only replaces the
-initial ln W, and the weight iteration with its flatness test stays the
-judge.
+initial ln W, and the weight iteration with its flatness test still decides
+when a run is done.
 """
 import os
 import time
@@ -178,7 +178,8 @@ def converge(instances, *, lnw_init=None, stratified=True, max_gpu_s=60.0, chunk
              device=0, tiles=KERNEL_AUTO, log=None):
     """Runs the batched weight iteration until every instance meets the
     reference's flatness test (width == N and D_KL < 1e-4) or max_gpu_s of
-    device time is spent.  Returns per-instance dicts with the converged (or
+    device time is spent (checked between calls of `chunk` iterations; the
+    chunk adapts to about 2-20 s of device time per call).  Returns per-instance dicts with the converged (or
     last) ln W, the status, and the device seconds spent while it was active."""
     b = Batch(instances, device=device, precision=REF32, tiles=tiles)
     try:
@@ -196,6 +197,16 @@ def converge(instances, *, lnw_init=None, stratified=True, max_gpu_s=60.0, chunk
             ms = b.iterate(chunk)
             st = b.status()
             gpu += ms / 1e3
+            # keep each call near 10 s of device time: once an instance's
+            # histogram spans all bins its NUPDATES grows 1 % per iteration, so
+            # the chunk is then kept short
+            ramp = any(x["reach_iter"] != 0xFFFFFFFF and not x["converged"] for x in st)
+            if ms > 20e3 and chunk > 1:
+                chunk = max(1, chunk // 4)
+            elif ms < 2e3 and chunk < 1000:
+                chunk *= 2
+            if ramp:
+                chunk = min(chunk, 8)
             # device time is charged to the instances that were active in the chunk
             act = [i for i in range(n) if not st0[i]["converged"]]
             for i in act:
