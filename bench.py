@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--delta", type=float, default=1.97)
     ap.add_argument("--cap-seeded", type=float, default=600.0)
     ap.add_argument("--cap-plain", type=float, default=600.0)
+    ap.add_argument("--save-lnw", default="", help="seeded: write the final ln W of each run "
+                    "to <path>_<run>.npy")
     ap.add_argument("--nup-factor", type=float, default=16.0,
                     help="seeded run: NUPDATES from the start = factor * N^2")
     ap.add_argument("--L", type=int, default=None, help="72 for c2, 128 for seeded")
@@ -399,6 +401,9 @@ def seeded_workload(M, args):
                 "last_D_KL": st["dkl"], "attempts": st["attempts"]}
 
     sm = {k: summary(v) for k, v in res.items()}
+    if args.save_lnw:
+        for k, v in res.items():
+            np.save(f"{args.save_lnw}_{k}.npy", v["lnw"])
     ratio = None
     if "seeded" in sm and "plain" in sm and sm["seeded"]["flat"]:
         r = sm["plain"]["gpu_s"] / sm["seeded"]["gpu_s"]
@@ -457,17 +462,24 @@ def c5_workload(M, args):
         ctx.compute_state()
         ctx.set_log_weight(lnw)
         _, ms = ctx.run(0, seed, therm, nup)
-        print(f"c5: production launch {ms:.1f} ms ({ctx.info.kernel=})", file=sys.stderr,
-              flush=True)
+        kernel_used = ctx.info.kernel
+        print(f"c5: production launch {ms:.1f} ms", file=sys.stderr, flush=True)
         deltas = np.round(np.linspace(1.90, 2.00, 11), 4)
         t0 = time.perf_counter()
         absM, chi, binder = ctx.reweight(deltas)
         rw_ms = (time.perf_counter() - t0) * 1e3
         joint0 = ctx.get_joint(0)
     i0 = int(np.argmin(np.abs(deltas - delta0)))
+    # the reference's evaluation: shift over all bins (produce2_multigpu.cu:391-400)
     wsh = lnw.copy()
     O.weight_shift(beta, delta0, wsh)
-    a_ref, c_ref = O.observables(beta, N, joint0, O.boltz(beta, delta0, wsh))
+    a_refall, c_refall = O.observables(beta, N, joint0, O.boltz(beta, delta0, wsh))
+    # the same long-double sums with the shift over the block's visited bins
+    S0 = O.joint_moments(N, joint0)[0]
+    Dg = np.arange(N + 1, dtype=np.float64)
+    expo = -beta * delta0 * Dg - lnw
+    bz = np.exp(expo - np.max(expo[S0 > 0]))
+    a_ref, c_ref = O.observables(beta, N, joint0, bz)
     del joint0
     obs = []
     for i, d in enumerate(deltas):
@@ -489,12 +501,14 @@ def c5_workload(M, args):
                    "therm": f"{args.therm_mult} N", "nupdates": f"{args.nup_mult} N",
                    "joint_bins_per_block": (N + 1) * (N + 2) // 2,
                    "weights": "surrogate seed (not converged)"},
-        "kernel": {1: "generic", 2: "team", 3: "lane"}.get(ctx.info.kernel, "?"),
+        "kernel": {1: "generic", 2: "team", 3: "lane"}.get(kernel_used, "?"),
         "reweight_ms": rw_ms,
         "observables": obs,
+        "visited_D_block0": [int(np.argmax(S0 > 0)), int(N - np.argmax(S0[::-1] > 0))],
         "oracle_block0": {"delta": delta0, "abs_M_device": float(absM[i0, 0]),
                           "abs_M_oracle": a_ref, "chi_device": float(chi[i0, 0]),
                           "chi_oracle": c_ref,
+                          "reference_all_bin_shift": {"abs_M": a_refall, "chi": c_refall},
                           "rel_diff_abs_M": abs(absM[i0, 0] - a_ref) / abs(a_ref),
                           "rel_diff_chi": abs(chi[i0, 0] - c_ref) / abs(c_ref)},
     }

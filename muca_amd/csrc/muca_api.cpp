@@ -172,8 +172,16 @@ int muca_create(const muca_params *params, muca_ctx **out)
     // 65536).  muca_autotune replaces this rule by measurement.
     const uint64_t ncu = (uint64_t)(cu > 0 ? cu : 256);
     if (p.kernel == MUCA_KERNEL_AUTO) {
-        const bool team_ok = p.L % 8 == 0 && p.L <= 256;
-        const bool lane_ok = p.L % 2 == 0 && p.L >= 4;
+        // eligibility as the launch decides it (layout in this device's LDS)
+        muca::McsArgs probe{};
+        probe.N = p.L * p.L;
+        probe.L = p.L;
+        probe.W = W;
+        probe.nup = 1;
+        const size_t lds_probe = (size_t)(lds > 0 ? lds : 65536);
+        muca::TeamLayout lay_probe;
+        const bool team_ok = muca::team_layout(probe, p.precision == MUCA_FP64, lds_probe, lay_probe);
+        const bool lane_ok = muca::lane_eligible(probe, p.precision == MUCA_FP64, lds_probe);
         if (team_ok && (uint64_t)W <= 128ull * ncu)
             c->cfg.kernel = MUCA_KERNEL_TEAM;
         else if (lane_ok)

@@ -399,11 +399,16 @@ __global__ __launch_bounds__(256) void k_reweight(const double *mom, const doubl
     const uint32_t id = blockIdx.x / blocks, b = blockIdx.x % blocks;
     const double bd = beta * deltas[id];
     const uint64_t nb = N + 1;
+    // The shift cancels in every ratio; the reference takes it over all bins
+    // (produce2_multigpu.cu:391-400), which underflows every visited bin to 0
+    // (and the ratios to 0/0) when the maximum sits at an unvisited D.  The
+    // maximum over the block's visited bins gives the same ratios wherever
+    // the reference's are finite, and finite ratios wherever a bin is visited.
+    const double *S = mom + (uint64_t)b * 4 * nb;
     double mx = -INFINITY;
     for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x)
-        mx = fmax(mx, -lnw[i] - bd * (double)i);
+        if (S[i] > 0) mx = fmax(mx, -lnw[i] - bd * (double)i);
     mx = block_reduce_max(mx, sh);
-    const double *S = mom + (uint64_t)b * 4 * nb;
     double z = 0, a1 = 0, a2 = 0, a4 = 0;
     for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
         const double bz = exp(-bd * (double)i - lnw[i] - mx);
