@@ -478,7 +478,7 @@ def c5_workload(M, args):
     S0 = O.joint_moments(N, joint0)[0]
     Dg = np.arange(N + 1, dtype=np.float64)
     expo = -beta * delta0 * Dg - lnw
-    bz = np.exp(expo - np.max(expo[S0 > 0]))
+    bz = np.where(S0 > 0, np.exp(np.minimum(expo - np.max(expo[S0 > 0]), 0.0)), 0.0)
     a_ref, c_ref = O.observables(beta, N, joint0, bz)
     del joint0
     obs = []

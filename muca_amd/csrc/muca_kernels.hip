@@ -403,7 +403,8 @@ __global__ __launch_bounds__(256) void k_reweight(const double *mom, const doubl
     // (produce2_multigpu.cu:391-400), which underflows every visited bin to 0
     // (and the ratios to 0/0) when the maximum sits at an unvisited D.  The
     // maximum over the block's visited bins gives the same ratios wherever
-    // the reference's are finite, and finite ratios wherever a bin is visited.
+    // the reference's are finite, and finite ratios wherever a bin is visited;
+    // unvisited bins are left out of the sums (their terms are 0 x exp(...)).
     const double *S = mom + (uint64_t)b * 4 * nb;
     double mx = -INFINITY;
     for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x)
@@ -411,6 +412,7 @@ __global__ __launch_bounds__(256) void k_reweight(const double *mom, const doubl
     mx = block_reduce_max(mx, sh);
     double z = 0, a1 = 0, a2 = 0, a4 = 0;
     for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
+        if (!(S[i] > 0)) continue;   // unvisited: contributes nothing (and may overflow)
         const double bz = exp(-bd * (double)i - lnw[i] - mx);
         z += bz * S[i];
         a1 += bz * S[nb + i];
