@@ -60,6 +60,8 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
                  sr[(uint64_t)(yu * L + x) * W] + sr[(uint64_t)(yd * L + x) * W];
         };
         uint32_t c0 = 0;
+        // a workgroup adds at most blockDim NUPDATES counts to one u32 bin
+        const bool may_wrap = (uint64_t)blockDim.x * a.nup >= (1ull << 32);
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             if (n == 0) continue;
@@ -108,9 +110,13 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
                         }
                         if (phase) {
                             if (LDSHIST) {
-                                // a u32 bin that wraps carries 2^32 into HBM
-                                if (atomicAdd(&hist[d], 1u) == 0xFFFFFFFFu)
-                                    atomicAdd((unsigned long long *)&a.hist[d], 1ull << 32);
+                                if (may_wrap) {
+                                    // a u32 bin that wraps carries 2^32 into HBM
+                                    if (atomicAdd(&hist[d], 1u) == 0xFFFFFFFFu)
+                                        atomicAdd((unsigned long long *)&a.hist[d], 1ull << 32);
+                                } else {
+                                    atomicAdd(&hist[d], 1u);
+                                }
                             }
                             else
                                 atomicAdd((unsigned long long *)&a.hist[d], 1ull);

@@ -124,6 +124,8 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
         uint32_t c0base = 0;
         int32_t de_lane = 0, dm_lane = 0;
+        // a workgroup adds at most 16 NUPDATES counts to one u32 bin
+        const bool may_wrap = (uint64_t)kTeams * a.nup >= (1ull << 32);
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             for (uint64_t rs = 0; rs < n; rs += Rw) {
@@ -214,22 +216,31 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     mv[r] = 0;
                 }
                 const int32_t d_start = d;
-                auto step = [&](auto lc, int r) {
+                // One chain step.  CHK = false: no memory access and no branch,
+                // a D outside the window only sets `bad` (the round is then
+                // replayed with CHK = true, which decides such a step exactly
+                // from the attempt's record in LDS).
+                auto step = [&](auto lc, auto chk, int r, uint32_t &ab, bool &bad) {
                     constexpr int l = decltype(lc)::value;
+                    constexpr bool CHK = decltype(chk)::value;
                     const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
                     const int32_t ddel = __builtin_amdgcn_sbfe((int32_t)mj, 16, 2);
                     const uint32_t off = (uint32_t)(d - dlo);
                     uint32_t acc = (mj >> (off & 15u)) & 1u;
                     const bool oob = off >= (uint32_t)kWin;
-                    if (__builtin_amdgcn_ballot_w64(oob)) {   // rare: D left the window
-                        const TeamRec<T> rc = rec[16 * r + l];
-                        T diff = rc.fde + wt[d + rc.ddel];
-                        diff = diff - wt[d];
-                        const uint32_t slow = (T)rc.logu < diff ? 1u : 0u;
-                        acc = oob ? slow : acc;
+                    if (CHK) {
+                        if (__builtin_amdgcn_ballot_w64(oob)) {   // D left the window
+                            const TeamRec<T> rc = rec[16 * r + l];
+                            T diff = rc.fde + wt[d + rc.ddel];
+                            diff = diff - wt[d];
+                            const uint32_t slow = (T)rc.logu < diff ? 1u : 0u;
+                            acc = oob ? slow : acc;
+                        }
+                    } else {
+                        bad = bad || oob;
                     }
                     d += (int32_t)acc * ddel;
-                    accb[r] |= acc << l;
+                    ab |= acc << l;
                     if (JOINT) {
                         const int32_t ij = row_bcast<l>(info[r]);
                         dv[r] = tl == l ? d : dv[r];
@@ -237,28 +248,64 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         mv[r] = tl == l ? m : mv[r];
                     }
                 };
-#define MUCA_STEP(LL) step(std::integral_constant<int, (LL)>(), r);
-#define MUCA_STEP_IF(LL) \
-    if ((LL) < left) step(std::integral_constant<int, (LL)>(), r);
+                using Fast = std::false_type;
+                using Checked = std::true_type;
+#define MUCA_STEPS(K, AB, BAD)                                                               \
+    if (left >= 16) {                                                                        \
+        step(std::integral_constant<int, 0>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 1>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 2>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 3>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 4>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 5>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 6>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 7>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 8>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 9>(), K(), r, AB, BAD);                             \
+        step(std::integral_constant<int, 10>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 11>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 12>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 13>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 14>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 15>(), K(), r, AB, BAD);                            \
+    } else {                                                                                 \
+        if (0 < left) step(std::integral_constant<int, 0>(), K(), r, AB, BAD);               \
+        if (1 < left) step(std::integral_constant<int, 1>(), K(), r, AB, BAD);               \
+        if (2 < left) step(std::integral_constant<int, 2>(), K(), r, AB, BAD);               \
+        if (3 < left) step(std::integral_constant<int, 3>(), K(), r, AB, BAD);               \
+        if (4 < left) step(std::integral_constant<int, 4>(), K(), r, AB, BAD);               \
+        if (5 < left) step(std::integral_constant<int, 5>(), K(), r, AB, BAD);               \
+        if (6 < left) step(std::integral_constant<int, 6>(), K(), r, AB, BAD);               \
+        if (7 < left) step(std::integral_constant<int, 7>(), K(), r, AB, BAD);               \
+        if (8 < left) step(std::integral_constant<int, 8>(), K(), r, AB, BAD);               \
+        if (9 < left) step(std::integral_constant<int, 9>(), K(), r, AB, BAD);               \
+        if (10 < left) step(std::integral_constant<int, 10>(), K(), r, AB, BAD);             \
+        if (11 < left) step(std::integral_constant<int, 11>(), K(), r, AB, BAD);             \
+        if (12 < left) step(std::integral_constant<int, 12>(), K(), r, AB, BAD);             \
+        if (13 < left) step(std::integral_constant<int, 13>(), K(), r, AB, BAD);             \
+        if (14 < left) step(std::integral_constant<int, 14>(), K(), r, AB, BAD);             \
+    }
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     const uint32_t j0 = 16 * r;
                     if (j0 >= cnt) break;
                     const uint32_t left = cnt - j0;
-                    if (left >= 16) {
-                        MUCA_STEP(0) MUCA_STEP(1) MUCA_STEP(2) MUCA_STEP(3)
-                        MUCA_STEP(4) MUCA_STEP(5) MUCA_STEP(6) MUCA_STEP(7)
-                        MUCA_STEP(8) MUCA_STEP(9) MUCA_STEP(10) MUCA_STEP(11)
-                        MUCA_STEP(12) MUCA_STEP(13) MUCA_STEP(14) MUCA_STEP(15)
-                    } else {
-                        MUCA_STEP_IF(0) MUCA_STEP_IF(1) MUCA_STEP_IF(2) MUCA_STEP_IF(3)
-                        MUCA_STEP_IF(4) MUCA_STEP_IF(5) MUCA_STEP_IF(6) MUCA_STEP_IF(7)
-                        MUCA_STEP_IF(8) MUCA_STEP_IF(9) MUCA_STEP_IF(10) MUCA_STEP_IF(11)
-                        MUCA_STEP_IF(12) MUCA_STEP_IF(13) MUCA_STEP_IF(14)
+                    const int32_t d_r0 = d, m_r0 = m;
+                    uint32_t ab = 0;
+                    bool bad = false;
+                    MUCA_STEPS(Fast, ab, bad)
+                    if (__builtin_amdgcn_ballot_w64(bad)) {   // rare: replay exactly
+                        if (bad) {
+                            d = d_r0;
+                            m = m_r0;
+                            ab = 0;
+                            bool unused = false;
+                            MUCA_STEPS(Checked, ab, unused)
+                        }
                     }
+                    accb[r] = ab;
                 }
-#undef MUCA_STEP_IF
-#undef MUCA_STEP
+#undef MUCA_STEPS
                 if (!JOINT) {
                     // D after each lane's own attempt: the chain's start plus the
                     // dDel of the accepted attempts up to it, counted from the
@@ -292,9 +339,14 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         }
                         if (phase) {
                             if (LDSHIST) {
-                                // a u32 bin that wraps carries 2^32 into HBM
-                                if (atomicAdd(&hist[dv[r]], 1u) == 0xFFFFFFFFu)
-                                    atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull << 32);
+                                if (may_wrap) {
+                                    // a u32 bin that wraps carries 2^32 into HBM
+                                    if (atomicAdd(&hist[dv[r]], 1u) == 0xFFFFFFFFu)
+                                        atomicAdd((unsigned long long *)&a.hist[dv[r]],
+                                                  1ull << 32);
+                                } else {
+                                    atomicAdd(&hist[dv[r]], 1u);
+                                }
                             }
                             else
                                 atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull);
