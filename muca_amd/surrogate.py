@@ -135,6 +135,34 @@ def generate_corpus(path, Ls=(16, 24, 32, 40, 48), Ts=(0.5, 0.6, 0.7, 0.8), walk
     return keep, gpu
 
 
+def extend_corpus(path, Ls, Ts, *, walkers=4096, factor=16.0, max_gpu_s=600.0, log=None):
+    """Closes the loop: a network trained on the corpus at `path` seeds new
+    (L, T) instances, which the batched engine converges with the seeded
+    protocol (trivial update, NUPDATES = factor N^2 from the start, the
+    reference's random start); the flat ones join the corpus.  Returns the
+    keys added and the device seconds spent."""
+    X, t, w = corpus(load_corpus(path))
+    net, _ = train(X, t, w)
+    ins, seeds, meta = [], [], []
+    for L in Ls:
+        for T in Ts:
+            ins.append(dict(L=L, walkers=walkers, seed=200 + len(ins), beta=1.0 / T,
+                            recursion=False, init_nup=equilibrium_nupdates(L, factor)))
+            seeds.append(seed_log_weight(net, L, 1.0 / T, DELTAS[0]))
+            meta.append((L, T))
+    net.close()
+    out, gpu = converge(ins, lnw_init=seeds, stratified=False, max_gpu_s=max_gpu_s, chunk=1,
+                        log=log)
+    d = dict(np.load(path))
+    added = []
+    for (L, T), o in zip(meta, out):
+        if o["status"]["converged"]:
+            d[f"L{L}_T{T}"] = o["lnw"]
+            added.append(f"L{L}_T{T}")
+    np.savez_compressed(path, **d)
+    return added, gpu
+
+
 def load_corpus(path=CORPUS_PATH, exclude=()):
     """Runs for corpus(): every stored (L, T) once per Delta of the grid."""
     d = np.load(path)
