@@ -29,7 +29,7 @@ a bounded sample of the same workload.
 D_KL < 1e-4, muca_delta_multigpu.cu:441-447) for an unseen (L, T, Delta) =
 (`--L`, 0.55, 1.97), 4096 walkers, trivial update.  "seeded": ln W from the
 surrogate (trained on the GPU on the committed corpus of converged weights,
-L <= 40), NUPDATES from the start = 16 N^2 (`--nup-factor`; the
+L <= 48), NUPDATES from the start = 16 N^2 (`--nup-factor`; the
 reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the
 reference's defaults).  Each run stops at its device-time cap; a run that
 hits it reports a lower bound.
