@@ -627,9 +627,17 @@ int muca_mlp_train(muca_mlp *g, uint32_t steps, uint32_t batch, float lr, float 
     MHIP(hipSetDevice(g->device));
     const uint32_t tiles = batch / kTile;
     MHIP(ensure_part(g, tiles));
-    hipEvent_t e0, e1;
-    MHIP(hipEventCreate(&e0));
-    MHIP(hipEventCreate(&e1));
+    struct Events {   // destroyed on every return path
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    MHIP(hipEventCreate(&ev.e[0]));
+    MHIP(hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     MHIP(hipEventRecord(e0, g->st));
     const uint32_t nbatches = g->n / batch;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -655,8 +663,6 @@ int muca_mlp_train(muca_mlp *g, uint32_t steps, uint32_t batch, float lr, float 
     MHIP(hipEventSynchronize(e1));
     float ms = 0;
     MHIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (gpu_ms) *gpu_ms = ms;
     if (loss_out) {
         float l = 0;
