@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--delta", type=float, default=1.97)
     ap.add_argument("--cap-seeded", type=float, default=600.0)
     ap.add_argument("--cap-plain", type=float, default=600.0)
+    ap.add_argument("--recursive", action="store_true",
+                    help="seeded workload: weight_recursion instead of the trivial update")
     ap.add_argument("--save-lnw", default="", help="seeded: write the final ln W of each run "
                     "to <path>_<run>.npy")
     ap.add_argument("--nup-factor", type=float, default=16.0,
@@ -384,13 +386,15 @@ def seeded_workload(M, args):
         nup0 = S.equilibrium_nupdates(L, args.nup_factor)
         print(f"seeded: trained on {len(X)} samples in {train_info['gpu_s']:.2f} GPU s, "
               f"NUPDATES from the start {nup0}", file=sys.stderr, flush=True)
-        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta, recursion=False,
-                                  init_nup=nup0)], lnw_init=[lnw0], stratified=False,
+        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta,
+                                  recursion=args.recursive, init_nup=nup0)],
+                            lnw_init=[lnw0], stratified=False,
                             max_gpu_s=args.cap_seeded, chunk=1, log=log("seeded"))
         res["seeded"] = out[0]
     if args.mode in ("both", "plain"):
-        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta, recursion=False,
-                                  init_nup=1)], stratified=False, max_gpu_s=args.cap_plain,
+        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta,
+                                  recursion=args.recursive, init_nup=1)], stratified=False,
+                            max_gpu_s=args.cap_plain,
                             chunk=50, log=log("plain"))
         res["plain"] = out[0]
 
@@ -428,7 +432,8 @@ def seeded_workload(M, args):
         "dtype": "int8 spins, f32 acceptance, f64 weight update; surrogate bf16 MFMA / f32",
         "data": "synthetic (random initial spins from the reference's Philox keying)",
         "config": {"workload": f"weight iteration to flat H, L={L}, T={args.T}, "
-                               f"Delta={args.delta}, {W} walkers, trivial update",
+                               f"Delta={args.delta}, {W} walkers, "
+                               f"{'recursive' if args.recursive else 'trivial'} update",
                    "seeded_nupdates": f"{args.nup_factor} N^2",
                    "L": L, "T": args.T, "delta": args.delta, "walkers": W,
                    "criterion": "width == N and D_KL < 1e-4",

@@ -1,0 +1,62 @@
+"""Full-size lattices on the GPU: L = 128 (the team kernel with lattices in
+HBM instead of LDS, the lane kernel with 64 KB of ln W in LDS) and L = 256
+(65 536 spins, 65 537 D-bins: ln W fits no LDS, the generic kernel).
+
+Every walker is an independent chain, so a sample of walkers pins the
+launch: the oracle replays walkers 0..7 (and the last walker) bit for bit;
+the rest of the launch is checked through properties that hold at any size
+(E, D, M recomputed from the final lattices equal the incremental ones; the
+histogram counts exactly walkers x NUPDATES entries)."""
+import numpy as np
+import pytest
+
+import muca_amd as M
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = [0, 1, 2, 3, 4, 5, 6, 7]
+
+
+def _weights(N, beta):
+    # a tilted, rough ln W so that acceptances are far from trivial
+    rng = np.random.default_rng(N)
+    lnw = -1.6 * beta * np.arange(N + 1) + np.cumsum(rng.normal(scale=0.7, size=N + 1))
+    return lnw - lnw[0]
+
+
+@pytest.mark.parametrize("L,W,kernel,prec", [
+    (128, 96, M.KERNEL_TEAM, M.REF32),
+    (128, 96, M.KERNEL_TEAM, M.FP64),
+    (128, 256, M.KERNEL_LANE, M.REF32),
+    (128, 96, M.KERNEL_GENERIC, M.REF32),
+    (256, 64, M.KERNEL_GENERIC, M.REF32),
+    (256, 64, M.KERNEL_GENERIC, M.FP64),
+], ids=["L128-team-f32", "L128-team-f64", "L128-lane-f32", "L128-generic-f32",
+        "L256-generic-f32", "L256-generic-f64"])
+def test_large_lattice_walker_sample(orc, L, W, kernel, prec):
+    N = L * L
+    beta = 2.0
+    lnw = _weights(N, beta)
+    therm, nup = N // 2, N + 3
+    with M.Context(L, W, kernel=kernel, precision=prec, beta=beta) as ctx:
+        ctx.init_lattice(mm=0, gpu=0, seed=41)
+        ctx.compute_state()
+        E0, D0, M0 = ctx.get_state()
+        ctx.set_log_weight(lnw)
+        hist, _ = ctx.run(2, 555, therm, nup)
+        glat = ctx.get_lattice()
+        gE, gD, gM = ctx.get_state()
+        ctx.compute_state()
+        rE, rD, rM = ctx.get_state()
+    assert int(hist.sum()) == W * nup
+    assert np.array_equal(gE, rE) and np.array_equal(gD, rD) and np.array_equal(gM, rM)
+    assert np.any(gD != D0)   # the walk moved
+    for w in SAMPLE + [W - 1]:
+        lat = orc.init_lattice(M.KEYING_WEIGHT, L, 1, seed=41, w0=w)
+        E, D, Mg = orc.state(L, lat)
+        assert (E[0], D[0], Mg[0]) == (E0[w], D0[w], M0[w])
+        orc.mcs(L, lat, E, D, Mg, keying=M.KEYING_WEIGHT, fp64=prec == M.FP64,
+                lnw32=lnw.astype(np.float32), lnw64=lnw, iter=2, seed=555, therm=therm,
+                nupdates=nup, beta=beta, w0=w)
+        assert np.array_equal(glat[:, w], lat[0]), w
+        assert (gE[w], gD[w]) == (E[0], D[0]), w
