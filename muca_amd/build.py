@@ -27,8 +27,10 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
-LIB_SOURCES = ["muca_kernels.hip", "muca_team.hip", "muca_lane.hip", "muca_batch.hip",
-               "muca_mlp.hip", "muca_api.cpp"]
+# largest first: the team kernel's instantiations dominate the build and are
+# split into two units so that the units compile in parallel
+LIB_SOURCES = ["muca_team_r7.hip", "muca_team_r10.hip", "muca_team.hip", "muca_lane.hip",
+               "muca_kernels.hip", "muca_mlp.hip", "muca_batch.hip", "muca_api.cpp"]
 CLI_SOURCES = {"multigpu_weight": "cli_weight.cpp", "produce_gpu": "cli_produce.cpp"}
 
 
@@ -40,9 +42,10 @@ def _run(cmd):
     return p
 
 
-def build_lib(verbose=False):
-    objs = []
+def build_lib(verbose=False, jobs=None):
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(os.path.join(PKG, "_build"), exist_ok=True)
+    cmds, objs = [], []
     for src in LIB_SOURCES:
         obj = os.path.join(PKG, "_build", src + ".o")
         cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
@@ -51,8 +54,11 @@ def build_lib(verbose=False):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)

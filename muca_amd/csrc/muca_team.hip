@@ -1,405 +1,23 @@
-// muca_team.hip -- the team update kernel: sixteen lanes (one DPP row) per
-// walker, four walkers per wavefront, sixteen per workgroup.
-//
-// Why: a walker is a strict serial chain, so with few walkers (4096 walkers
-// fill only 64 wavefronts) one-lane-per-walker leaves most SIMDs idle and
-// every attempt waits on a chain of dependent memory loads.  The reference's
-// even-L site order is column-parity stripes (muca_delta_multigpu.cu:173):
-// a half-row of L/2 same-parity sites contains no two neighbours, and every
-// neighbour of a half-row was last written by an earlier half-row.  So per
-// half-row the team
-//   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
-//      the proposed spin, dE, dDel, (T)(-dE*beta), the acceptance log and an
-//      acceptance bitmask over a 16-wide window of D values around the D the
-//      row starts from;
-//   B. runs only the D chain serially: attempt j's mask is broadcast from
-//      its lane with a DPP row broadcast and the accept bit is read at the
-//      current D -- no memory access on the chain.  An attempt whose D has
-//      left the window (more than 8 net moves in one half-row) takes an
-//      exact slow path from LDS;
-//   C. writes the accepted spins, sums dE and dM and fills the histogram.
-// Every acceptance is computed with the reference's operations and rounding:
-//   diff = ((T)(-dE*beta) + lnW[D+dDel]) - lnW[D];  accept iff diff >= 0 or
-//   acclog(r1) < diff.  Because acclog(r1) < 0 always, that is exactly
-//   acclog(r1) < diff.  Results are bit-identical to the one-lane kernel.
-// Requires L % 8 == 0 (whole Philox blocks per half-row) and L <= 256.
-//
-// LDS (dynamic): per team the Philox words [2][L/2] and attempt records
-// [L/2] (slow path); the acceptance-log table; ln W with guard entries on
-// each side; the D histogram (u32, when it fits); the workgroup's lattices
-// [N][16] int8 (when they fit).
-#include "kernels.h"
-
-#include "acclog.h"
-#include "philox.h"
+// muca_team.hip -- the team update kernel's launch side: LDS layout,
+// dispatch over the instantiations (compiled in muca_team_r7.hip and
+// muca_team_r10.hip so that the build can run them in parallel), and the
+// batched engine's persistent team-tile kernel.
+#include "team_run.h"
 
 namespace muca {
 
-namespace {
-
-constexpr int kTeam = 16;
-constexpr int kTeams = 16;   // per workgroup of 256 lanes
-constexpr int kWin = 16;     // D values per acceptance mask
-constexpr int kGuard = 24;   // ln W guard entries on each side (> kWin / 2 + 2)
-
-template <typename T>
-struct TeamRec {
-    T fde;          // (T)(-dE * beta)
-    float logu;     // acceptance log of r1
-    int32_t ddel;   // new^2 - old^2
-};
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ int32_t team_sum(int32_t v)
-{
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, kTeam);
-    return v;
-}
-
-// value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L)
-template <int L>
-__device__ __forceinline__ int32_t row_bcast(int32_t v)
-{
-    return __builtin_amdgcn_update_dpp(v, v, 0x150 + L, 0xf, 0xf, false);
-}
-
-}  // namespace
-
-// NR: rounds of 16 attempts per half-row, NR * 16 >= L / 2
-// One workgroup's sixteen walkers wbase .. wbase+15 of the launch described
-// by a (all 256 lanes of the workgroup call it together).
-template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
-__device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay, uint64_t wbase,
-                                         unsigned char *lds)
-{
-    typedef typename std::conditional<FP64, double, float>::type T;
-    __syncthreads();   // LDS may still be read by a previous call's flush
-    T *wt = (T *)(lds + lay.off_w) + kGuard;   // valid for [-kGuard, N + kGuard]
-    uint32_t *hist = (uint32_t *)(lds + lay.off_hist);
-    int8_t *latl = (int8_t *)(lds + lay.off_lat);
-    double *logtab = (double *)(lds + lay.off_tab);
-    const int tid = threadIdx.x, team = tid >> 4, tl = tid & 15;
-    const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
-    unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
-    uint32_t *rng = (uint32_t *)tbase;                          // [2][Rw]
-    TeamRec<T> *rec = (TeamRec<T> *)(tbase + lay.off_rec);      // [Rw]
-
-    const uint64_t W = a.W;
-    const uint64_t w = wbase + team;
-    const bool active = w < W;
-
-    for (int i = (int)tid - kGuard; i <= (int)N + kGuard; i += blockDim.x) {
-        T v = (T)0;
-        if (i >= 0 && i <= (int)N) v = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
-        wt[i] = v;
-    }
-    if (LDSHIST)
-        for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = 0;
-    acclog_table_to_lds(logtab, tid, blockDim.x);
-    if (LATLDS) {
-        for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
-            const uint32_t site = e >> 4, t = e & 15;
-            latl[t * lay.lat_stride + site] =
-                (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
-        }
-    }
-    __syncthreads();
-
-    if (active) {
-        auto sp = [&](uint32_t site) -> int8_t & {
-            return LATLDS ? latl[team * lay.lat_stride + site] : a.lat[(uint64_t)site * W + w];
-        };
-        const uint32_t wid = a.w_offset + (uint32_t)w;
-        const uint32_t k1a = KEYING == 0 ? 2u * wid : wid;
-        const uint32_t k1b = 0xdecafbadu;
-        const uint32_t k3a = KEYING == 0 ? 2u * wid + 50000u : wid + 40000u;
-        const uint32_t k3b = KEYING == 0 ? 0x11cafeadu : 0xdecafeadu;
-        int32_t d = a.D[w], e = a.E[w], m = a.M[w];
-        uint64_t *jbase = nullptr;
-        if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
-        uint32_t c0base = 0;
-        int32_t de_lane = 0, dm_lane = 0;
-        // a workgroup adds at most 16 NUPDATES counts to one u32 bin
-        const bool may_wrap = (uint64_t)kTeams * a.nup >= (1ull << 32);
-        for (int phase = 0; phase < 2; phase++) {
-            const uint64_t n = phase ? a.nup : a.therm;
-            for (uint64_t rs = 0; rs < n; rs += Rw) {
-                const uint32_t cnt = (uint32_t)min((uint64_t)Rw, n - rs);
-                const uint32_t pos0 = (uint32_t)(rs % N);
-                const uint32_t h = pos0 / NH;
-                const uint32_t y = (pos0 - h * NH) / Rw;
-                // ---- A1: Philox blocks of this half-row, one (block, key) per lane
-                const uint32_t nblk = (cnt + 3) / 4;
-                const uint32_t blk0 = c0base + (uint32_t)(rs / 4);
-                for (uint32_t task = tl; task < 2 * nblk; task += kTeam) {
-                    const uint32_t b = task >> 1, key = task & 1;
-                    const philox4 r = philox4x32<R>(blk0 + b + 1, a.seed, a.iter, 0xBADC0DEDu,
-                                                    key ? k3a : k1a, key ? k3b : k1b);
-                    uint32_t *dst = rng + key * Rw + 4 * b;
-                    dst[0] = r.v[0];
-                    dst[1] = r.v[1];
-                    dst[2] = r.v[2];
-                    dst[3] = r.v[3];
-                }
-                wave_sync();
-                // ---- A2: attempt j = 16 r + tl on this lane, everything in registers
-                const int32_t dlo = d - kWin / 2;     // mask bit k <-> D = dlo + k
-                T win[kWin + 2];                      // lnW[dlo - 1 .. dlo + kWin]
-#pragma unroll
-                for (int k = 0; k < kWin + 2; k++) win[k] = wt[dlo - 1 + k];
-                const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
-                uint32_t mask[NR];
-                int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
-                // Branch-free over the rounds so that the loads of all rounds
-                // can be in flight together: a lane past the end of the
-                // half-row works on a clamped (in-bounds) attempt and keeps
-                // nothing of it.
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const uint32_t j = 16 * r + tl;
-                    const bool valid = j < cnt;
-                    const uint32_t jc = valid ? j : cnt - 1;
-                    const uint32_t x = 2 * jc + h;
-                    const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
-                    const uint32_t xl = (x == 0) ? L - 1 : x - 1;
-                    const int32_t old_s = sp(y * L + x);
-                    const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
-                                       sp(yd * L + x);
-                    const uint32_t r1 = rng[jc], r3 = rng[Rw + jc];
-                    int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
-                    new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
-                    const int32_t ds = new_s - old_s;
-                    const int32_t dE = -ds * nb;
-                    const int32_t ddel = new_s * new_s - old_s * old_s;
-                    const T fde = (T)(-(double)dE * a.beta);
-                    // the correctly rounded acceptance log (a 16-wide mask needs it
-                    // for every attempt; screening each of the 16 comparisons with
-                    // the hardware log costs more than it saves here)
-                    const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
-                    const T tlog = (T)logu;
-                    uint32_t mk = 0;
-#pragma unroll
-                    for (int k = kWin - 1; k >= 0; k--) {
-                        const T wa = ddel > 0 ? win[k + 2] : (ddel < 0 ? win[k] : win[k + 1]);
-                        T diff = fde + wa;
-                        diff = diff - win[k + 1];
-                        mk = mk + mk + (tlog < diff ? 1u : 0u);
-                    }
-                    // bits 0..15: accept at D = dlo + k; bits 16..17: dDel
-                    mask[r] = mk | (valid ? ((uint32_t)ddel & 3u) << 16 : 0u);
-                    if (JOINT)
-                        info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
-                                        : ((8 << 2) | (2 << 7));
-                    else
-                        info[r] = valid ? ((dE + 8) << 2) | ((ds + 2) << 7) : ((8 << 2) | (2 << 7));
-                    if (valid) {
-                        TeamRec<T> rc;
-                        rc.fde = fde;
-                        rc.logu = logu;
-                        rc.ddel = ddel;
-                        rec[j] = rc;
-                    }
-                }
-                wave_sync();
-                // ---- B: the D chain
-                uint32_t accb[NR];
-                int32_t dv[NR], mv[NR];
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    accb[r] = 0;
-                    dv[r] = 0;
-                    mv[r] = 0;
-                }
-                const int32_t d_start = d;
-                // One chain step.  CHK = false: no memory access and no branch,
-                // a D outside the window only sets `bad` (the round is then
-                // replayed with CHK = true, which decides such a step exactly
-                // from the attempt's record in LDS).
-                auto step = [&](auto lc, auto chk, int r, uint32_t &ab, bool &bad) {
-                    constexpr int l = decltype(lc)::value;
-                    constexpr bool CHK = decltype(chk)::value;
-                    const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
-                    const int32_t ddel = __builtin_amdgcn_sbfe((int32_t)mj, 16, 2);
-                    const uint32_t off = (uint32_t)(d - dlo);
-                    uint32_t acc = (mj >> (off & 15u)) & 1u;
-                    const bool oob = off >= (uint32_t)kWin;
-                    if (CHK) {
-                        if (__builtin_amdgcn_ballot_w64(oob)) {   // D left the window
-                            const TeamRec<T> rc = rec[16 * r + l];
-                            T diff = rc.fde + wt[d + rc.ddel];
-                            diff = diff - wt[d];
-                            const uint32_t slow = (T)rc.logu < diff ? 1u : 0u;
-                            acc = oob ? slow : acc;
-                        }
-                    } else {
-                        bad = bad || oob;
-                    }
-                    d += (int32_t)acc * ddel;
-                    ab |= acc << l;
-                    if (JOINT) {
-                        const int32_t ij = row_bcast<l>(info[r]);
-                        dv[r] = tl == l ? d : dv[r];
-                        m += (int32_t)acc * (((ij >> 7) & 7) - 2);
-                        mv[r] = tl == l ? m : mv[r];
-                    }
-                };
-                using Fast = std::false_type;
-                using Checked = std::true_type;
-#define MUCA_STEPS(K, AB, BAD)                                                               \
-    if (left >= 16) {                                                                        \
-        step(std::integral_constant<int, 0>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 1>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 2>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 3>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 4>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 5>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 6>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 7>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 8>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 9>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 10>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 11>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 12>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 13>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 14>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 15>(), K(), r, AB, BAD);                            \
-    } else {                                                                                 \
-        if (0 < left) step(std::integral_constant<int, 0>(), K(), r, AB, BAD);               \
-        if (1 < left) step(std::integral_constant<int, 1>(), K(), r, AB, BAD);               \
-        if (2 < left) step(std::integral_constant<int, 2>(), K(), r, AB, BAD);               \
-        if (3 < left) step(std::integral_constant<int, 3>(), K(), r, AB, BAD);               \
-        if (4 < left) step(std::integral_constant<int, 4>(), K(), r, AB, BAD);               \
-        if (5 < left) step(std::integral_constant<int, 5>(), K(), r, AB, BAD);               \
-        if (6 < left) step(std::integral_constant<int, 6>(), K(), r, AB, BAD);               \
-        if (7 < left) step(std::integral_constant<int, 7>(), K(), r, AB, BAD);               \
-        if (8 < left) step(std::integral_constant<int, 8>(), K(), r, AB, BAD);               \
-        if (9 < left) step(std::integral_constant<int, 9>(), K(), r, AB, BAD);               \
-        if (10 < left) step(std::integral_constant<int, 10>(), K(), r, AB, BAD);             \
-        if (11 < left) step(std::integral_constant<int, 11>(), K(), r, AB, BAD);             \
-        if (12 < left) step(std::integral_constant<int, 12>(), K(), r, AB, BAD);             \
-        if (13 < left) step(std::integral_constant<int, 13>(), K(), r, AB, BAD);             \
-        if (14 < left) step(std::integral_constant<int, 14>(), K(), r, AB, BAD);             \
-    }
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const uint32_t j0 = 16 * r;
-                    if (j0 >= cnt) break;
-                    const uint32_t left = cnt - j0;
-                    const int32_t d_r0 = d, m_r0 = m;
-                    uint32_t ab = 0;
-                    bool bad = false;
-                    MUCA_STEPS(Fast, ab, bad)
-                    if (__builtin_amdgcn_ballot_w64(bad)) {   // rare: replay exactly
-                        if (bad) {
-                            d = d_r0;
-                            m = m_r0;
-                            ab = 0;
-                            bool unused = false;
-                            MUCA_STEPS(Checked, ab, unused)
-                        }
-                    }
-                    accb[r] = ab;
-                }
-#undef MUCA_STEPS
-                if (!JOINT) {
-                    // D after each lane's own attempt: the chain's start plus the
-                    // dDel of the accepted attempts up to it, counted from the
-                    // team's 16-bit slices of two ballots
-                    const uint32_t sh = (uint32_t)(tid & 48);
-                    const uint32_t upto = (2u << tl) - 1u;
-                    int32_t drun = d_start;
-#pragma unroll
-                    for (int r = 0; r < NR; r++) {
-                        const int32_t my = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
-                        const uint32_t up = (uint32_t)(__builtin_amdgcn_ballot_w64(my > 0) >> sh) &
-                                            0xffffu & accb[r];
-                        const uint32_t dn = (uint32_t)(__builtin_amdgcn_ballot_w64(my < 0) >> sh) &
-                                            0xffffu & accb[r];
-                        dv[r] = drun + __builtin_popcount(up & upto) - __builtin_popcount(dn & upto);
-                        drun += __builtin_popcount(up) - __builtin_popcount(dn);
-                    }
-                }
-                // ---- C: accepted spins, dE / dM sums, histogram
-                int32_t de = 0, dm = 0;
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const uint32_t j = 16 * r + tl;
-                    if (j < cnt) {
-                        if ((accb[r] >> tl) & 1u) {
-                            const int32_t ds = ((info[r] >> 7) & 7) - 2;
-                            int8_t &s = sp(y * L + 2 * j + h);
-                            s = (int8_t)(s + ds);
-                            de += ((info[r] >> 2) & 31) - 8;
-                            dm += ds;
-                        }
-                        if (phase) {
-                            if (LDSHIST) {
-                                if (may_wrap) {
-                                    // a u32 bin that wraps carries 2^32 into HBM
-                                    if (atomicAdd(&hist[dv[r]], 1u) == 0xFFFFFFFFu)
-                                        atomicAdd((unsigned long long *)&a.hist[dv[r]],
-                                                  1ull << 32);
-                                } else {
-                                    atomicAdd(&hist[dv[r]], 1u);
-                                }
-                            }
-                            else
-                                atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull);
-                            if (JOINT) {
-                                const int64_t dd = dv[r];
-                                const int64_t mm = min(max((int64_t)mv[r], -dd), dd);
-                                atomicAdd((unsigned long long *)&jbase[(mm + dd * dd + 2 * dd) / 2],
-                                          1ull);
-                            }
-                        }
-                    }
-                }
-                de_lane += de;
-                dm_lane += dm;
-                if (!LATLDS) __threadfence_block();
-                wave_sync();
-            }
-            c0base += (uint32_t)((n + 3) / 4);
-        }
-        // dE and dM of accepted moves were summed per lane; one reduction
-        // over the team's sixteen lanes per launch.  In joint mode the
-        // chain itself tracks M (it is needed per attempt).
-        e += team_sum(de_lane);
-        const int32_t dm_team = team_sum(dm_lane);
-        if (!JOINT) m += dm_team;
-        if (tl == 0) {
-            a.D[w] = d;
-            a.E[w] = e;
-            a.M[w] = m;
-        }
-    }
-    __syncthreads();
-    if (LDSHIST) {
-        for (uint32_t i = tid; i <= N; i += blockDim.x) {
-            const uint32_t c = hist[i];
-            if (c) atomicAdd((unsigned long long *)&a.hist[i], (unsigned long long)c);
-        }
-    }
-    if (LATLDS) {
-        for (uint32_t e2 = tid; e2 < N * kTeams; e2 += blockDim.x) {
-            const uint32_t site = e2 >> 4, t = e2 & 15;
-            if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[t * lay.lat_stride + site];
-        }
-    }
-}
-
-template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
-__global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
-{
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    team_run<R, FP64, KEYING, LATLDS, LDSHIST, JOINT, NR>(a, lay, (uint64_t)blockIdx.x * kTeams,
-                                                         lds);
-}
+#define MUCA_TEAM_EXTERN(R, FP64, KEYING)                                                \
+    extern template hipError_t launch_team_r<R, FP64, KEYING>(hipStream_t, const McsArgs &, \
+                                                              const TeamLayout &);
+MUCA_TEAM_EXTERN(7, false, 0)
+MUCA_TEAM_EXTERN(7, false, 1)
+MUCA_TEAM_EXTERN(7, true, 0)
+MUCA_TEAM_EXTERN(7, true, 1)
+MUCA_TEAM_EXTERN(10, false, 0)
+MUCA_TEAM_EXTERN(10, false, 1)
+MUCA_TEAM_EXTERN(10, true, 0)
+MUCA_TEAM_EXTERN(10, true, 1)
+#undef MUCA_TEAM_EXTERN
 
 bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
 {
@@ -435,43 +53,6 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     if (lay.lat_lds) off += al((size_t)stride * kTeams);
     lay.bytes = off;
     return off <= lds_max;
-}
-
-template <int R, bool FP64, int KEYING, int NR>
-static hipError_t launch_team_nr(hipStream_t st, const McsArgs &a, const TeamLayout &lay)
-{
-    const dim3 grid((unsigned)((a.W + kTeams - 1) / kTeams)), block(kTeams * kTeam);
-    const bool joint = a.joint != nullptr;
-    void (*k)(McsArgs, TeamLayout) = nullptr;
-#define MUCA_TEAM_PICK(LL, LH)                                                  \
-    k = joint ? k_mcs_team<R, FP64, KEYING, LL, LH, true, NR>                   \
-              : k_mcs_team<R, FP64, KEYING, LL, LH, false, NR>;
-    if (lay.lat_lds && lay.lds_hist) {
-        MUCA_TEAM_PICK(true, true)
-    } else if (lay.lat_lds) {
-        MUCA_TEAM_PICK(true, false)
-    } else if (lay.lds_hist) {
-        MUCA_TEAM_PICK(false, true)
-    } else {
-        MUCA_TEAM_PICK(false, false)
-    }
-#undef MUCA_TEAM_PICK
-    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lay.bytes);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, grid, block, lay.bytes, st, a, lay);
-    return hipGetLastError();
-}
-
-template <int R, bool FP64, int KEYING>
-static hipError_t launch_team_r(hipStream_t st, const McsArgs &a, const TeamLayout &lay)
-{
-    const uint32_t rounds = (lay.row + 15) / 16;
-    if (rounds <= 1) return launch_team_nr<R, FP64, KEYING, 1>(st, a, lay);
-    if (rounds <= 2) return launch_team_nr<R, FP64, KEYING, 2>(st, a, lay);
-    if (rounds <= 3) return launch_team_nr<R, FP64, KEYING, 3>(st, a, lay);
-    if (rounds <= 4) return launch_team_nr<R, FP64, KEYING, 4>(st, a, lay);
-    return launch_team_nr<R, FP64, KEYING, 8>(st, a, lay);
 }
 
 hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
