@@ -29,7 +29,7 @@ a bounded sample of the same workload.
 D_KL < 1e-4, muca_delta_multigpu.cu:441-447) for an unseen (L, T, Delta) =
 (`--L`, 0.55, 1.97), 4096 walkers, trivial update.  "seeded": ln W from the
 surrogate (trained on the GPU on the committed corpus of converged weights,
-L <= 48), NUPDATES from the start = 16 N^2 (`--nup-factor`; the
+L = 16 .. 64, T = 0.5 .. 0.8), NUPDATES from the start = 24 N^2 (`--nup-factor`; the
 reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the
 reference's defaults).  Each run stops at its device-time cap; a run that
 hits it reports a lower bound.
@@ -210,7 +210,7 @@ def main():
                     help="seeded workload: weight_recursion instead of the trivial update")
     ap.add_argument("--save-lnw", default="", help="seeded: write the final ln W of each run "
                     "to <path>_<run>.npy")
-    ap.add_argument("--nup-factor", type=float, default=16.0,
+    ap.add_argument("--nup-factor", type=float, default=24.0,
                     help="seeded run: NUPDATES from the start = factor * N^2")
     ap.add_argument("--L", type=int, default=None, help="72 for c2, 128 for seeded")
     ap.add_argument("--walkers", type=int, default=0)
