@@ -1,6 +1,6 @@
 // muca_team.hip -- the team update kernel's launch side: LDS layout,
-// dispatch over the instantiations (compiled in muca_team_r7.hip and
-// muca_team_r10.hip so that the build can run them in parallel), and the
+// dispatch over the instantiations (compiled in muca_team_r{7,10}_{f32,f64}.hip
+// so that the build can run them in parallel), and the
 // batched engine's persistent team-tile kernel.
 #include "team_run.h"
 
