@@ -48,8 +48,9 @@ hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a);
 // team kernel (muca_team.hip): LDS layout of one workgroup of 16 walkers
 struct TeamLayout {
     uint32_t row;         // L/2 attempts per half-row
-    uint32_t team_bytes;  // per-team Philox words, attempt records and D sequence
-    uint32_t off_rec, off_dseq;  // offsets inside a team's area
+    uint32_t team_bytes;  // per-team Philox ring and attempt records
+    uint32_t off_rec;     // offset of the attempt records inside a team's area
+    uint32_t ring;        // Philox blocks per key held in a team's ring
     uint32_t off_team, off_tab, off_w, off_hist, off_lat;
     uint32_t lat_stride;  // LDS bytes per walker lattice (>= N, bank-spread)
     size_t bytes;         // dynamic LDS per workgroup

@@ -27,8 +27,10 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     const size_t recsz = fp64 ? sizeof(TeamRec<double>) : sizeof(TeamRec<float>);
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     lay.row = (uint32_t)row;
-    lay.off_rec = (uint32_t)al(2 * row * 4);
-    lay.off_dseq = 0;
+    // Philox ring: a half-row needs row/4 blocks per key and a generation pass
+    // may run up to 7 blocks ahead of them
+    lay.ring = (uint32_t)(row / 4 + 8);
+    lay.off_rec = (uint32_t)al(2 * (size_t)lay.ring * 16);
     lay.team_bytes = (uint32_t)(lay.off_rec + al(row * recsz));
     size_t off = 0;
     lay.off_team = 0;

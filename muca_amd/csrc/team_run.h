@@ -24,8 +24,8 @@
 //   acclog(r1) < diff.  Results are bit-identical to the one-lane kernel.
 // Requires L % 8 == 0 (whole Philox blocks per half-row) and L <= 256.
 //
-// LDS (dynamic): per team the Philox words [2][L/2] and attempt records
-// [L/2] (slow path); the acceptance-log table; ln W with guard entries on
+// LDS (dynamic): per team a ring of Philox blocks [2 keys][L/8 + 8][4] and
+// attempt records [L/2] (slow path); the acceptance-log table; ln W with guard entries on
 // each side; the D histogram (u32, when it fits); the workgroup's lattices
 // [N][16] int8 (when they fit).
 #pragma once
@@ -88,7 +88,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     const int tid = threadIdx.x, team = tid >> 4, tl = tid & 15;
     const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
     unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
-    uint32_t *rng = (uint32_t *)tbase;                          // [2][Rw]
+    uint32_t *rng = (uint32_t *)tbase;                          // [2][ring][4]
     TeamRec<T> *rec = (TeamRec<T> *)(tbase + lay.off_rec);      // [Rw]
 
     const uint64_t W = a.W;
@@ -125,6 +125,8 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         uint64_t *jbase = nullptr;
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
         uint32_t c0base = 0;
+        const uint32_t RB = lay.ring;
+        uint32_t gen = 0, gen_slot = 0, cur_slot = 0;   // next block to make, its slot, block blk0's slot
         int32_t de_lane = 0, dm_lane = 0;
         // a workgroup adds at most 16 NUPDATES counts to one u32 bin
         const bool may_wrap = (uint64_t)kTeams * a.nup >= (1ull << 32);
@@ -135,19 +137,33 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 const uint32_t pos0 = (uint32_t)(rs % N);
                 const uint32_t h = pos0 / NH;
                 const uint32_t y = (pos0 - h * NH) / Rw;
-                // ---- A1: Philox blocks of this half-row, one (block, key) per lane
+                // ---- A1: the walker's Philox stream (block b uses counter b + 1;
+                // the stream runs on across half-rows and phases) is generated
+                // eight blocks x two keys per pass into a ring of `RB` blocks per
+                // key, so no lane idles when a half-row's block count is not a
+                // multiple of eight.  Blocks [blk0, blk0 + nblk) serve this
+                // half-row; a pass may run up to 7 blocks ahead, and RB >=
+                // nblk + 8 keeps those from overwriting blocks still in use.
                 const uint32_t nblk = (cnt + 3) / 4;
                 const uint32_t blk0 = c0base + (uint32_t)(rs / 4);
-                for (uint32_t task = tl; task < 2 * nblk; task += kTeam) {
-                    const uint32_t b = task >> 1, key = task & 1;
-                    const philox4 r = philox4x32<R>(blk0 + b + 1, a.seed, a.iter, 0xBADC0DEDu,
+                while (gen < blk0 + nblk) {
+                    const uint32_t b = gen + (uint32_t)(tl >> 1), key = tl & 1;
+                    const philox4 r = philox4x32<R>(b + 1, a.seed, a.iter, 0xBADC0DEDu,
                                                     key ? k3a : k1a, key ? k3b : k1b);
-                    uint32_t *dst = rng + key * Rw + 4 * b;
+                    uint32_t slot = gen_slot + (uint32_t)(tl >> 1);
+                    slot = slot >= RB ? slot - RB : slot;
+                    uint32_t *dst = rng + (key * RB + slot) * 4;
                     dst[0] = r.v[0];
                     dst[1] = r.v[1];
                     dst[2] = r.v[2];
                     dst[3] = r.v[3];
+                    gen += 8;
+                    gen_slot = gen_slot + 8 >= RB ? gen_slot + 8 - RB : gen_slot + 8;
                 }
+                // ring slot of this half-row's first block (blocks are contiguous
+                // from one half-row to the next, across the phase change too)
+                const uint32_t slot0 = cur_slot;
+                cur_slot = cur_slot + nblk >= RB ? cur_slot + nblk - RB : cur_slot + nblk;
                 wave_sync();
                 // ---- A2: attempt j = 16 r + tl on this lane, everything in registers
                 const int32_t dlo = d - kWin / 2;     // mask bit k <-> D = dlo + k
@@ -172,7 +188,10 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t old_s = sp(y * L + x);
                     const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
                                        sp(yd * L + x);
-                    const uint32_t r1 = rng[jc], r3 = rng[Rw + jc];
+                    uint32_t rslot = slot0 + (jc >> 2);
+                    rslot = rslot >= RB ? rslot - RB : rslot;
+                    const uint32_t widx = rslot * 4 + (jc & 3);
+                    const uint32_t r1 = rng[widx], r3 = rng[RB * 4 + widx];
                     int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
                     new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
                     const int32_t ds = new_s - old_s;
