@@ -10,13 +10,14 @@
 // half-row the team
 //   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
 //      the proposed spin, dE, dDel, (T)(-dE*beta), the acceptance log and an
-//      acceptance bitmask over a 16-wide window of D values around the D the
+//      acceptance bitmask over a window of kWin D values around the D the
 //      row starts from;
 //   B. runs only the D chain serially: attempt j's mask is broadcast from
 //      its lane with a DPP row broadcast and the accept bit is read at the
-//      current D -- no memory access on the chain.  An attempt whose D has
-//      left the window (more than 8 net moves in one half-row) takes an
-//      exact slow path from LDS;
+//      current D -- no memory access and no branch on the chain.  A round of
+//      16 steps in which D left the window (more than kWin/2 net moves from
+//      the row's start) is replayed with an exact per-step slow path from
+//      LDS;
 //   C. writes the accepted spins, sums dE and dM and fills the histogram.
 // Every acceptance is computed with the reference's operations and rounding:
 //   diff = ((T)(-dE*beta) + lnW[D+dDel]) - lnW[D];  accept iff diff >= 0 or
@@ -40,7 +41,13 @@ namespace team_detail {
 
 constexpr int kTeam = 16;
 constexpr int kTeams = 16;   // per workgroup of 256 lanes
-constexpr int kWin = 16;     // D values per acceptance mask
+// D values per acceptance mask.  Narrower windows cut the mask work per
+// attempt and make the exact replay of a round (D left the window) more
+// frequent.  Measured update-kernel times (L = 72, 4096 walkers, weight
+// iterations from ln W = 0): 16 -> 0.590 ms, 12 -> 0.572, 8 -> 0.540,
+// 6 -> 0.540; on flat L = 64 weights at T = 0.55: 6 -> 44.4 attempts/ns,
+// 8 -> 51.9, 12 -> 54.3.
+constexpr int kWin = 8;
 constexpr int kGuard = 24;   // ln W guard entries on each side (> kWin / 2 + 2)
 
 template <typename T>
@@ -198,7 +205,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t dE = -ds * nb;
                     const int32_t ddel = new_s * new_s - old_s * old_s;
                     const T fde = (T)(-(double)dE * a.beta);
-                    // the correctly rounded acceptance log (a 16-wide mask needs it
+                    // the correctly rounded acceptance log (a kWin-wide mask needs it
                     // for every attempt; screening each of the 16 comparisons with
                     // the hardware log costs more than it saves here)
                     const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
