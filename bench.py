@@ -36,11 +36,12 @@ hits it reports a lower bound.
 
 `--workload c5`: large production, L = 256 (65 536 spins), 16 384 walkers,
 T = 0.5, produce keying and FP64 acceptance, ln W from the surrogate, one
-launch of therm = N and NUPDATES = N per walker into ten per-block joint
-histograms H(D, M) (ten x 17.2 GB in HBM), then FP64 reweighting on the
-device to <|M|>, chi and the Binder cumulant over a Delta grid with
-jackknife errors; block 0 at Delta = 1.98 is checked against the CPU
-oracle's long-double observables().
+launch of therm = N and NUPDATES = N per walker into ten per-block sets of
+exact per-D moments (count, sum |M|, sum M^2, sum M^4; 3.1 MB each) or,
+with `--observables joint`, the reference's joint histograms H(D, M) (ten x
+17.2 GB in HBM), then FP64 reweighting on the device to <|M|>, chi and the
+Binder cumulant over a Delta grid with jackknife errors; block 0 at
+Delta = 1.98 is checked against the CPU oracle's long-double sums.
 
     python bench.py [--steps K] [--warmup W] [--walkers 4096] [--L 72]
     python bench.py --workload batch96 [--walkers 4096]
@@ -201,6 +202,8 @@ def main():
     ap.add_argument("--therm-mult", type=int, default=1, help="c5: therm = mult * N")
     ap.add_argument("--nup-mult", type=int, default=1, help="c5: NUPDATES = mult * N")
     ap.add_argument("--blocks", type=int, default=10, help="c5: jackknife blocks")
+    ap.add_argument("--observables", choices=["moments", "joint"], default="moments",
+                    help="c5: per-D moments (default) or the reference's joint H(D,M)")
     ap.add_argument("--mode", choices=["both", "seeded", "plain"], default="both")
     ap.add_argument("--T", type=float, default=0.55)
     ap.add_argument("--delta", type=float, default=1.97)
@@ -462,30 +465,44 @@ def c5_workload(M, args):
     therm, nup = args.therm_mult * N, args.nup_mult * N
     print(f"c5: L={L} walkers={W} blocks={B} therm={therm} NUPDATES={nup}", file=sys.stderr,
           flush=True)
-    with M.Context(L, W, keying=M.KEYING_PRODUCE, precision=M.FP64, joint_blocks=B) as ctx:
+    joint = args.observables == "joint"
+    obs_mode = M.OBS_JOINT if joint else M.OBS_MOMENTS
+    with M.Context(L, W, keying=M.KEYING_PRODUCE, precision=M.FP64, joint_blocks=B,
+                   observables=obs_mode) as ctx:
         ctx.init_lattice(mm=0, gpu=0, seed=seed)
         ctx.compute_state()
         ctx.set_log_weight(lnw)
         _, ms = ctx.run(0, seed, therm, nup)
-        kernel_used = ctx.info.kernel
+        kernel_used, block_used = ctx.info.kernel, ctx.info.block
         print(f"c5: production launch {ms:.1f} ms", file=sys.stderr, flush=True)
         deltas = np.round(np.linspace(1.90, 2.00, 11), 4)
         t0 = time.perf_counter()
         absM, chi, binder = ctx.reweight(deltas)
         rw_ms = (time.perf_counter() - t0) * 1e3
-        joint0 = ctx.get_joint(0)
+        if joint:
+            joint0 = ctx.get_joint(0)
+        else:
+            mom0 = ctx.get_moments(0)
     i0 = int(np.argmin(np.abs(deltas - delta0)))
-    # the reference's evaluation: shift over all bins (produce2_multigpu.cu:391-400)
     wsh = lnw.copy()
     O.weight_shift(beta, delta0, wsh)
-    a_refall, c_refall = O.observables(beta, N, joint0, O.boltz(beta, delta0, wsh))
-    # the same long-double sums with the shift over the block's visited bins
-    S0 = O.joint_moments(N, joint0)[0]
+    bz_all = O.boltz(beta, delta0, wsh)
+    if joint:
+        S0 = O.joint_moments(N, joint0)[0]
+    else:
+        S0 = np.array([float(x) for x in mom0[0]])
+    # the long-double sums with the shift over the block's visited bins
     Dg = np.arange(N + 1, dtype=np.float64)
     expo = -beta * delta0 * Dg - lnw
     bz = np.where(S0 > 0, np.exp(np.minimum(expo - np.max(expo[S0 > 0]), 0.0)), 0.0)
-    a_ref, c_ref = O.observables(beta, N, joint0, bz)
-    del joint0
+    if joint:
+        # the reference's evaluation: shift over all bins (produce2_multigpu.cu:391-400)
+        a_refall, c_refall = O.observables(beta, N, joint0, bz_all)
+        a_ref, c_ref = O.observables(beta, N, joint0, bz)
+        del joint0
+    else:
+        a_refall, c_refall = O.observables_moments(beta, N, mom0[0], mom0[1], mom0[2], bz_all)
+        a_ref, c_ref = O.observables_moments(beta, N, mom0[0], mom0[1], mom0[2], bz)
     obs = []
     for i, d in enumerate(deltas):
         am, ae = M.jackknife(absM[i] / N)
@@ -499,14 +516,19 @@ def c5_workload(M, args):
         "unit": "attempts/ns",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int8 spins, f64 acceptance, u64 joint histograms, f64 reweighting",
+        "dtype": "int8 spins, f64 acceptance, " +
+                 ("u64 joint histograms" if joint else "exact per-D moments (u64/u128)") +
+                 ", f64 reweighting",
         "data": "synthetic (random initial spins from the reference's Philox keying)",
         "config": {"workload": f"large production, L={L}, {W} walkers, T=0.5, "
                                f"{B} jackknife blocks", "L": L, "walkers": W,
                    "therm": f"{args.therm_mult} N", "nupdates": f"{args.nup_mult} N",
-                   "joint_bins_per_block": (N + 1) * (N + 2) // 2,
+                   "observables": args.observables,
+                   "accumulator_words_per_block": (N + 1) * (N + 2) // 2 if joint
+                   else 6 * (N + 1),
                    "weights": "surrogate seed (not converged)"},
         "kernel": {1: "generic", 2: "team", 3: "lane"}.get(kernel_used, "?"),
+        "block": block_used,
         "reweight_ms": rw_ms,
         "observables": obs,
         "visited_D_block0": [int(np.argmax(S0 > 0)), int(N - np.argmax(S0[::-1] > 0))],

@@ -30,6 +30,7 @@ extern "C" {
 #define MUCA_E_NOMEM (-3)    /* device allocation failed */
 #define MUCA_E_STATE (-4)    /* call out of order (e.g. no weights set) */
 #define MUCA_E_IO (-5)       /* file could not be opened or parsed */
+#define MUCA_E_RANGE (-6)    /* a count exceeded what its accumulator holds exactly */
 
 /* acceptance precision */
 #define MUCA_REF32 0   /* float weights and diff: multigpu_weight (muca_delta_multigpu.cu:775,810) */
@@ -45,6 +46,10 @@ extern "C" {
 #define MUCA_KERNEL_TEAM 2     /* sixteen lanes per walker, LDS-resident lattices (L % 8 == 0) */
 #define MUCA_KERNEL_LANE 3     /* one lane per walker, even L, table-free, screened acceptance */
 
+/* what the produce path accumulates per walker block */
+#define MUCA_OBS_JOINT 0    /* joint H(D,M), produce2_multigpu.cu:489 */
+#define MUCA_OBS_MOMENTS 1  /* per D: count, sum |M|, sum M^2, sum M^4 (exact integers) */
+
 typedef struct muca_ctx muca_ctx;
 
 typedef struct muca_params {
@@ -58,7 +63,11 @@ typedef struct muca_params {
     uint32_t joint_blocks; /* produce path: 0 = no joint H(D,M); B >= 1 = one joint
                               histogram per contiguous block of walkers (jackknife) */
     int32_t kernel;      /* MUCA_KERNEL_* */
-    uint32_t block;      /* workgroup size of the one-lane kernels (0: 256; 64..1024) */
+    uint32_t block;      /* workgroup size of the one-lane kernels (0: chosen from
+                            walkers and compute units; else 64..1024) */
+    int32_t observables; /* produce path with joint_blocks > 0: MUCA_OBS_JOINT keeps
+                            the reference's joint H(D,M); MUCA_OBS_MOMENTS keeps
+                            exact per-D moments instead (6 (N+1) words per block) */
 } muca_params;
 
 typedef struct muca_info {
@@ -125,6 +134,15 @@ int muca_autotune(muca_ctx *ctx, uint64_t attempts, int32_t *kernel, uint32_t *b
 int muca_get_hist(muca_ctx *ctx, uint64_t *hist, uint32_t nbins, float *copy_ms);
 /* joint histogram of one walker block: (N+1)(N+2)/2 bins at (M+D*D+2*D)/2 */
 int muca_get_joint(muca_ctx *ctx, uint32_t block, uint64_t *joint, uint64_t nbins);
+/* Per-D moments of one walker block (MUCA_OBS_MOMENTS), 6 (N+1) uint64 as six
+ * arrays over D: S0 = sum_M H(D,M), S1 = sum_M |M| H, S2 = sum_M M^2 H as
+ * (low, high) words of a 128-bit integer, S4 = sum_M M^4 H likewise.  They
+ * equal the moments of the joint histogram exactly; the reference reads the
+ * same sums off H(D,M) in observables() (produce2_multigpu.cu:498-518).
+ * Exact while every bin of the last launch holds fewer than 2^32 entries;
+ * MUCA_E_RANGE otherwise (here and in muca_reweight).  In this mode muca_run
+ * takes nupdates < 2^32. */
+int muca_get_moments(muca_ctx *ctx, uint32_t block, uint64_t *moments, uint64_t nwords);
 
 /* ---- host-side weight iteration pieces (O(N), run between launches) ---- */
 /* hist_range (muca_delta_multigpu.cu:573-596) */
@@ -142,7 +160,7 @@ int muca_normalise(uint32_t N, double *lnw);
 int muca_jackknife(const double *x, uint32_t J, double *avg, double *err);
 
 /* ---- produce path on the device ---- */
-/* Reweighting of the joint histograms to canonical <|M|>, chi and the Binder
+/* Reweighting of the joint histograms (or per-D moments) to canonical <|M|>, chi and the Binder
  * cumulant at ndelta crystal fields (produce2_multigpu.cu:293-358,391-400,
  * 498-518), in FP64 on the device with a max-shifted log-sum-exp.  Uses the
  * ln W last set with muca_set_log_weight.  Outputs are [ndelta][blocks]. */

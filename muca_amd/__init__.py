@@ -22,6 +22,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 REF32, FP64 = 0, 1
 KEYING_WEIGHT, KEYING_PRODUCE = 0, 1
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_TEAM, KERNEL_LANE = 0, 1, 2, 3
+OBS_JOINT, OBS_MOMENTS = 0, 1
 SEED = 100                       # muca_delta_multigpu.cu:19
 
 _u8p = ctypes.POINTER(ctypes.c_int8)
@@ -40,7 +41,8 @@ class Params(ctypes.Structure):
                 ("device", ctypes.c_int32), ("rounds", ctypes.c_uint32),
                 ("beta", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("keying", ctypes.c_int32), ("joint_blocks", ctypes.c_uint32),
-                ("kernel", ctypes.c_int32), ("block", ctypes.c_uint32)]
+                ("kernel", ctypes.c_int32), ("block", ctypes.c_uint32),
+                ("observables", ctypes.c_int32)]
 
 
 class Info(ctypes.Structure):
@@ -120,6 +122,8 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_uint32), _f32p]),
     "muca_get_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
                                       ctypes.c_uint64]),
+    "muca_get_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
+                                        ctypes.c_uint64]),
     "muca_hist_range": (ctypes.c_int, [_u64p, ctypes.c_uint32, _i32p, _i32p]),
     "muca_kull_leibler": (ctypes.c_double, [_u64p, ctypes.c_uint32]),
     "muca_update_trivial": (ctypes.c_int, [_u64p, ctypes.c_uint32, _f64p]),
@@ -178,10 +182,11 @@ class Context:
     main(), muca_delta_multigpu.cu:177-205 / produce2_multigpu.cu:247-275)."""
 
     def __init__(self, L, walkers=0, device=0, *, rounds=7, beta=2.0, precision=REF32,
-                 keying=KEYING_WEIGHT, joint_blocks=0, kernel=KERNEL_AUTO, block=0):
+                 keying=KEYING_WEIGHT, joint_blocks=0, kernel=KERNEL_AUTO, block=0,
+                 observables=OBS_JOINT):
         self._h = ctypes.c_void_p()
         p = Params(L, walkers, device, rounds, beta, precision, keying, joint_blocks, kernel,
-                   block)
+                   block, observables)
         _check(lib().muca_create(ctypes.byref(p), ctypes.byref(self._h)))
         info = Info()
         _check(lib().muca_query(self._h, ctypes.byref(info)))
@@ -259,6 +264,15 @@ class Context:
         a = np.empty(n, np.uint64)
         _check(lib().muca_get_joint(self._h, block, _p(a, _u64p), n))
         return a
+
+    def get_moments(self, block=0):
+        """Exact per-D moments of one walker block (OBS_MOMENTS) as Python
+        integers: arrays S0, S1, S2, S4 over D (object dtype, 128-bit sums)."""
+        n = 6 * (self.N + 1)
+        a = np.empty(n, np.uint64)
+        _check(lib().muca_get_moments(self._h, block, _p(a, _u64p), n))
+        w = a.reshape(6, self.N + 1).astype(object)
+        return w[0], w[1], w[2] + (w[3] << 64), w[4] + (w[5] << 64)
 
     def reweight(self, deltas):
         """(<|M|>, chi, Binder) arrays of shape (len(deltas), joint_blocks)."""

@@ -3,7 +3,7 @@
 //
 //   produce_gpu L first_gpu last_gpu range jacks therm_mult nup_mult delta0 gap
 //               [--walkers=W] [--seed=S] [--rounds=7|10] [--jack-blocks=B]
-//               [--weight-file=PATH] [--outdir=DIR] [--host-observables]
+//               [--weight-file=PATH] [--outdir=DIR] [--host-observables | --moments]
 //
 // Positional arguments, stdout and the observables file follow
 // produce2_multigpu.cu:91-363: ln W is read from
@@ -14,7 +14,9 @@
 // Delta = delta0 + gap*(l+1)/range with a jackknife over GPUs.
 // Differences, all deliberate:
 //  - reweighting runs in FP64 on the device from per-D moments of H(D,M)
-//    (--host-observables keeps the reference's long-double host loop);
+//    (--host-observables keeps the reference's long-double host loop;
+//    --moments accumulates exact per-D moments in the update kernel instead
+//    of H(D,M), 6 (N+1) words per block in place of (N+1)(N+2)/2);
 //  - --jack-blocks=B splits each GPU's walkers into B contiguous blocks and
 //    jackknifes over all blocks (the reference's one-GPU jackknife is 0/0);
 //  - a missing weight file is an error (the reference prints "!error!" and
@@ -104,7 +106,7 @@ int main(int argc, char *argv[])
     if (argc < 10)
         die("usage: produce_gpu L first_gpu last_gpu range jacks therm_mult nup_mult delta0 gap "
             "[--walkers=W] [--seed=S] [--rounds=7|10] [--jack-blocks=B] [--weight-file=PATH] "
-            "[--outdir=DIR] [--host-observables]");
+            "[--outdir=DIR] [--host-observables | --moments]");
     const unsigned L = (unsigned)stoi(argv[1]);
     const int first_gpu = stoi(argv[2]), last_gpu = stoi(argv[3]);
     if (last_gpu < first_gpu) die("last_gpu < first_gpu");
@@ -115,7 +117,7 @@ int main(int argc, char *argv[])
     const double beta = 1.0 / 0.5, delta_i = delta, gap = atof(argv[9]);
 
     uint32_t walkers = 0, seed = 100, rounds = 7, jack_blocks = 0;
-    bool host_obs = false;
+    bool host_obs = false, moments = false;
     string outdir = ".", wpath;
     for (int i = 10; i < argc; i++) {
         string a = argv[i], v;
@@ -126,8 +128,10 @@ int main(int argc, char *argv[])
         else if (opt(a, "weight-file", v)) wpath = v;
         else if (opt(a, "outdir", v)) outdir = v;
         else if (a == "--host-observables") host_obs = true;
+        else if (a == "--moments") moments = true;
         else die("unknown option " + a);
     }
+    if (host_obs && moments) die("--host-observables needs the joint histogram, not --moments");
     const unsigned N = L * L;
     const string fss = outdir + "/fss/L" + argv[1];
     mkdirs(fss);
@@ -159,6 +163,7 @@ int main(int argc, char *argv[])
         p.precision = MUCA_FP64;
         p.keying = MUCA_KEYING_PRODUCE;
         p.joint_blocks = B;
+        p.observables = moments ? MUCA_OBS_MOMENTS : MUCA_OBS_JOINT;
         check(muca_create(&p, &ctx[g]), "muca_create");
         check(muca_init_lattice(ctx[g], 0, g, seed), "muca_init_lattice");  // :126-140
         check(muca_set_log_weight(ctx[g], log_weight.data(), N + 1), "set_log_weight");

@@ -17,6 +17,7 @@ struct McsArgs {
     const uint32_t *order, *naver;
     uint64_t *hist;          // N+1 bins (zeroed by the caller)
     uint64_t *joint;         // nullable: joint_blocks * joint_stride bins
+    uint64_t *moms;          // nullable: joint_blocks * kMomWords * (N+1) exact per-D moments
     uint64_t W;              // walkers in the lattice
     uint32_t N, L;
     uint32_t w_offset;       // logical id of walker 0 (for keys)
@@ -119,6 +120,7 @@ __device__ __forceinline__ McsArgs batch_mcs_args(const BatchArgs &b, const Batc
     a.naver = nullptr;
     a.hist = b.hist + I.w_off;
     a.joint = nullptr;
+    a.moms = nullptr;
     a.W = I.W;
     a.N = I.N;
     a.L = I.L;
@@ -147,6 +149,17 @@ hipError_t launch_batch_update(hipStream_t st, const BatchArgs &b);
 hipError_t launch_acclog(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *out);
 
+// Exact per-D moments of a walker block, accumulated by the update kernel in
+// place of the joint histogram: kMomWords arrays of N+1 uint64 each,
+// S0 = count, S1 = sum |M|, then S2 = sum M^2 and S4 = sum M^4 each as three
+// words w0, w1, w2 with S = w0 + w1 2^32 + w2 2^64.  Every add puts less than
+// 2^32 into each word, so the words are exact while a bin holds fewer than
+// 2^32 entries (S0 < 2^32); launch_moments_fp64 flags bins beyond that.
+constexpr int kMomWords = 8;
+// FP64 moments [block][4][N+1] (k_joint_moments' layout); *range_flag (zeroed
+// by the caller) is set to 1 if some bin has S0 >= 2^32
+hipError_t launch_moments_fp64(hipStream_t st, const uint64_t *moms, uint32_t N,
+                               uint32_t blocks, double *mom, uint32_t *range_flag);
 // joint H(D,M) -> per-D moments (S0, S1, S2, S4) for each block, FP64
 hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t N,
                                 uint64_t stride, uint32_t blocks, double *mom);

@@ -45,8 +45,10 @@ struct muca_ctx {
     uint32_t *order = nullptr, *naver = nullptr;
     uint64_t *hist = nullptr;
     uint64_t *joint = nullptr;
+    uint64_t *moms = nullptr;   // MUCA_OBS_MOMENTS: blocks * kMomWords * (N+1)
     uint64_t joint_stride = 0, block_walkers = 0;
     double *mom = nullptr, *deltas = nullptr, *rw_out = nullptr;
+    uint32_t *rw_flag = nullptr;
     uint32_t rw_cap = 0;
     bool have_weights = false;
     muca::Launch cfg{};
@@ -82,7 +84,8 @@ void free_ctx(muca_ctx *c)
     if (!c) return;
     if (c->p.device >= 0) hipSetDevice(c->p.device);
     void *bufs[] = {c->lat, c->E, c->D, c->M, c->lnw32, c->lnw64, c->order, c->naver,
-                    c->hist, c->joint, c->mom, c->deltas, c->rw_out};
+                    c->hist, c->joint, c->moms, c->mom, c->deltas, c->rw_out,
+                    c->rw_flag};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -135,6 +138,13 @@ int muca_create(const muca_params *params, muca_ctx **out)
     if (p.keying != MUCA_KEYING_WEIGHT && p.keying != MUCA_KEYING_PRODUCE)
         return fail(MUCA_E_ARG, "keying must be MUCA_KEYING_WEIGHT or MUCA_KEYING_PRODUCE");
     if (!(p.beta > 0) || !std::isfinite(p.beta)) return fail(MUCA_E_ARG, "beta must be > 0");
+    if (p.observables != MUCA_OBS_JOINT && p.observables != MUCA_OBS_MOMENTS)
+        return fail(MUCA_E_ARG, "observables must be MUCA_OBS_JOINT or MUCA_OBS_MOMENTS");
+    const bool moments = p.joint_blocks > 0 && p.observables == MUCA_OBS_MOMENTS;
+    if (moments && p.precision != MUCA_FP64)
+        return fail(MUCA_E_ARG, "per-D moments are accumulated in MUCA_FP64 mode only");
+    if (moments && p.kernel != MUCA_KERNEL_AUTO && p.kernel != MUCA_KERNEL_GENERIC)
+        return fail(MUCA_E_ARG, "per-D moments are accumulated by the generic kernel only");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (p.device < 0 || p.device >= ndev)
@@ -182,7 +192,9 @@ int muca_create(const muca_params *params, muca_ctx **out)
         muca::TeamLayout lay_probe;
         const bool team_ok = muca::team_layout(probe, p.precision == MUCA_FP64, lds_probe, lay_probe);
         const bool lane_ok = muca::lane_eligible(probe, p.precision == MUCA_FP64, lds_probe);
-        if (team_ok && (uint64_t)W <= 128ull * ncu)
+        if (moments)
+            c->cfg.kernel = MUCA_KERNEL_GENERIC;
+        else if (team_ok && (uint64_t)W <= 128ull * ncu)
             c->cfg.kernel = MUCA_KERNEL_TEAM;
         else if (lane_ok)
             c->cfg.kernel = MUCA_KERNEL_LANE;
@@ -196,6 +208,13 @@ int muca_create(const muca_params *params, muca_ctx **out)
         return fail(MUCA_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
     }
     c->cfg.block = p.block ? (int)p.block : 256;
+    if (p.block == 0 && c->cfg.kernel == MUCA_KERNEL_GENERIC) {
+        // one walker per lane and no LDS lattice: the grid should reach every
+        // compute unit before workgroups grow (16 384 walkers at 256 lanes
+        // would occupy 64 of 256 compute units)
+        while (c->cfg.block > 64 && (uint64_t)W < (uint64_t)c->cfg.block * ncu)
+            c->cfg.block /= 2;
+    }
     if (p.block == 0 && c->cfg.kernel == MUCA_KERNEL_LANE && (uint64_t)W >= 1024ull * ncu) {
         // Enough walkers for two 512-lane workgroups per compute unit; when
         // the lane kernel's LDS (ln W and the histogram) admits only one
@@ -224,9 +243,14 @@ int muca_create(const muca_params *params, muca_ctx **out)
             free_ctx(c);
             return fail(MUCA_E_ARG, "joint_blocks exceeds walkers");
         }
-        c->joint_stride = (N + 1) * (N + 2) / 2;
         c->block_walkers = (W + p.joint_blocks - 1) / p.joint_blocks;
-        rc = dev_alloc(&c->joint, c->joint_stride * p.joint_blocks, "joint histogram");
+        if (p.observables == MUCA_OBS_MOMENTS) {
+            rc = dev_alloc(&c->moms, (uint64_t)muca::kMomWords * (N + 1) * p.joint_blocks,
+                           "per-D moments");
+        } else {
+            c->joint_stride = (N + 1) * (N + 2) / 2;
+            rc = dev_alloc(&c->joint, c->joint_stride * p.joint_blocks, "joint histogram");
+        }
     }
     if (rc) {
         free_ctx(c);
@@ -245,6 +269,8 @@ int muca_create(const muca_params *params, muca_ctx **out)
     if (e == hipSuccess) e = hipMemset(c->E, 0, W * 4);
     if (e == hipSuccess) e = hipMemset(c->D, 0, W * 4);
     if (e == hipSuccess) e = hipMemset(c->M, 0, W * 4);
+    if (e == hipSuccess && c->moms)
+        e = hipMemset(c->moms, 0, (uint64_t)muca::kMomWords * (N + 1) * p.joint_blocks * 8);
     if (e != hipSuccess) {
         free_ctx(c);
         return fail(MUCA_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -385,6 +411,12 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     const bool joint = c->joint && c->p.precision == MUCA_FP64 && nup > 0;
     if (joint)
         HIP_TRY(hipMemsetAsync(c->joint, 0, c->joint_stride * c->p.joint_blocks * 8, c->st));
+    const bool moments = c->moms && nup > 0;
+    if (moments && (nup >> 32))
+        return fail(MUCA_E_ARG, "per-D moments take nupdates < 2^32 per launch");
+    if (moments)
+        HIP_TRY(hipMemsetAsync(c->moms, 0,
+                               (uint64_t)muca::kMomWords * (N + 1) * c->p.joint_blocks * 8, c->st));
     muca::McsArgs a{};
     a.lat = c->lat;
     a.E = c->E;
@@ -396,6 +428,7 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     a.naver = c->naver;
     a.hist = c->hist;
     a.joint = joint ? c->joint : nullptr;
+    a.moms = moments ? c->moms : nullptr;
     a.W = c->W;
     a.N = c->N;
     a.L = c->p.L;
@@ -477,10 +510,12 @@ int muca_autotune(muca_ctx *c, uint64_t attempts, int32_t *kernel, uint32_t *blo
     };
     std::vector<Cand> cands;
     muca::TeamLayout lay{};
-    if (muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay)) cands.push_back({MUCA_KERNEL_TEAM, 256});
-    if (muca::lane_eligible(a, c->cfg.fp64, c->cfg.lds_max))
+    const bool moments = c->moms != nullptr;   // accumulated by the generic kernel only
+    if (!moments && muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay))
+        cands.push_back({MUCA_KERNEL_TEAM, 256});
+    if (!moments && muca::lane_eligible(a, c->cfg.fp64, c->cfg.lds_max))
         for (int b : {256, 512, 1024}) cands.push_back({MUCA_KERNEL_LANE, b});
-    cands.push_back({MUCA_KERNEL_GENERIC, 256});
+    for (int b : {64, 128, 256}) cands.push_back({MUCA_KERNEL_GENERIC, b});
     hipError_t e = save_restore(true);
     float best = 0;
     Cand pick = {c->cfg.kernel, c->cfg.block};
@@ -543,17 +578,47 @@ int muca_get_joint(muca_ctx *c, uint32_t block, uint64_t *dst, uint64_t nbins)
     return MUCA_OK;
 }
 
+int muca_get_moments(muca_ctx *c, uint32_t block, uint64_t *dst, uint64_t nwords)
+{
+    if (!c || !dst) return fail(MUCA_E_ARG, "NULL argument");
+    if (!c->moms) return fail(MUCA_E_STATE, "context has no per-D moments");
+    if (block >= c->p.joint_blocks) return fail(MUCA_E_ARG, "block out of range");
+    const uint64_t nb = (uint64_t)c->N + 1, n = (uint64_t)muca::kMomWords * nb;
+    if (nwords != 6 * nb) return fail(MUCA_E_ARG, "nwords must be 6 (N+1)");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    std::vector<uint64_t> w(n);
+    HIP_TRY(hipMemcpy(w.data(), c->moms + (uint64_t)block * n, n * 8, hipMemcpyDeviceToHost));
+    // limb words w0 + w1 2^32 + w2 2^64 -> (low, high) words of a 128-bit integer
+    for (uint64_t d = 0; d < nb; d++) {
+        if (w[d] >> 32) return fail(MUCA_E_RANGE, "a D bin holds 2^32 or more entries");
+        dst[d] = w[d];
+        dst[nb + d] = w[nb + d];
+        for (int k = 0; k < 2; k++) {
+            const uint64_t *l = &w[(2 + 3 * k) * nb + d];
+            const unsigned __int128 v = (unsigned __int128)l[0] +
+                                        ((unsigned __int128)l[nb] << 32) +
+                                        ((unsigned __int128)l[2 * nb] << 64);
+            dst[(2 + 2 * k) * nb + d] = (uint64_t)v;
+            dst[(3 + 2 * k) * nb + d] = (uint64_t)(v >> 64);
+        }
+    }
+    return MUCA_OK;
+}
+
 int muca_reweight(muca_ctx *c, const double *deltas, uint32_t ndelta, double *absM,
                   double *chi, double *binder)
 {
     if (!c || !deltas || ndelta == 0) return fail(MUCA_E_ARG, "bad arguments");
-    if (!c->joint) return fail(MUCA_E_STATE, "context has no joint histogram");
+    if (!c->joint && !c->moms)
+        return fail(MUCA_E_STATE, "context has no joint histogram or per-D moments");
     if (!c->have_weights) return fail(MUCA_E_STATE, "muca_set_log_weight was not called");
     HIP_TRY(hipSetDevice(c->p.device));
     const uint32_t B = c->p.joint_blocks;
     const uint64_t nb = c->N + 1;
     if (!c->mom) {
         int rc = dev_alloc(&c->mom, (uint64_t)B * 4 * nb, "moments");
+        if (!rc) rc = dev_alloc(&c->rw_flag, 1, "range flag");
         if (rc) return rc;
     }
     if (ndelta > c->rw_cap) {
@@ -568,12 +633,21 @@ int muca_reweight(muca_ctx *c, const double *deltas, uint32_t ndelta, double *ab
         c->rw_cap = ndelta;
     }
     HIP_TRY(hipMemcpyAsync(c->deltas, deltas, ndelta * 8, hipMemcpyHostToDevice, c->st));
-    HIP_TRY(muca::launch_joint_moments(c->st, c->joint, c->N, c->joint_stride, B, c->mom));
+    if (c->joint)
+        HIP_TRY(muca::launch_joint_moments(c->st, c->joint, c->N, c->joint_stride, B, c->mom));
+    else {
+        HIP_TRY(hipMemsetAsync(c->rw_flag, 0, 4, c->st));
+        HIP_TRY(muca::launch_moments_fp64(c->st, c->moms, c->N, B, c->mom, c->rw_flag));
+    }
     HIP_TRY(muca::launch_reweight(c->st, c->mom, c->lnw64, c->N, B, c->p.beta, c->deltas,
                                   ndelta, c->rw_out));
     std::vector<double> o((size_t)ndelta * B * 3);
+    uint32_t range = 0;
     HIP_TRY(hipMemcpyAsync(o.data(), c->rw_out, o.size() * 8, hipMemcpyDeviceToHost, c->st));
+    if (!c->joint)
+        HIP_TRY(hipMemcpyAsync(&range, c->rw_flag, 4, hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
+    if (range) return fail(MUCA_E_RANGE, "a D bin holds 2^32 or more entries");
     for (size_t i = 0; i < (size_t)ndelta * B; i++) {
         if (absM) absM[i] = o[3 * i];
         if (chi) chi[i] = o[3 * i + 1];

@@ -139,7 +139,66 @@ __device__ __forceinline__ double lnw_at<double>(const McsArgs &a, int32_t i)
     return a.lnw64[i];
 }
 
-template <int R, bool FP64, int KEYING, bool LDSHIST, bool JOINT>
+// A 128-bit value (vl, vh < 2^32) added to three limb words w0 + w1 2^32 +
+// w2 2^64 with non-returning atomics (no carry to detect: each word gains
+// less than 2^32 per add).
+__device__ __forceinline__ void atomic_add_limbs(uint64_t *w0, uint64_t *w1, uint64_t *w2,
+                                                 uint64_t vl, uint64_t vh)
+{
+    const uint64_t l0 = vl & 0xFFFFFFFFull, l1 = vl >> 32;
+    if (l0) atomicAdd((unsigned long long *)w0, (unsigned long long)l0);
+    if (l1) atomicAdd((unsigned long long *)w1, (unsigned long long)l1);
+    if (vh) atomicAdd((unsigned long long *)w2, (unsigned long long)vh);
+}
+
+// A walker's production attempts at one D, summed in registers and added to
+// its block's per-D moments when D changes: count, sum |M|, and 128-bit sums
+// of M^2 and M^4 (|M| <= N <= 2^16, so M^4 <= 2^64 needs a high word too).
+struct MomRun {
+    uint64_t n, s1, s2l, s2h, s4l, s4h;
+    int32_t d;
+};
+
+__device__ __forceinline__ void mom_add(MomRun &r, int32_t m)
+{
+    const uint64_t am = (uint64_t)(m < 0 ? -m : m);
+    const uint64_t m2 = am * am;
+    const uint64_t m4l = m2 * m2, m4h = __umul64hi(m2, m2);
+    r.n += 1;
+    r.s1 += am;
+    r.s2l += m2;
+    r.s2h += r.s2l < m2 ? 1u : 0u;
+    r.s4l += m4l;
+    r.s4h += m4h + (r.s4l < m4l ? 1u : 0u);
+}
+
+__device__ __forceinline__ void mom_flush(const MomRun &r, uint64_t *mb, uint64_t nb)
+{
+    if (r.n == 0) return;
+    const uint64_t d = (uint64_t)r.d;
+    atomicAdd((unsigned long long *)&mb[d], (unsigned long long)r.n);
+    atomicAdd((unsigned long long *)&mb[nb + d], (unsigned long long)r.s1);
+    // a run holds at most NUPDATES < 2^32 attempts, so the high words are < 2^32
+    atomic_add_limbs(&mb[2 * nb + d], &mb[3 * nb + d], &mb[4 * nb + d], r.s2l, r.s2h);
+    atomic_add_limbs(&mb[5 * nb + d], &mb[6 * nb + d], &mb[7 * nb + d], r.s4l, r.s4h);
+}
+
+// n entries of D histogram bin d: LDS (uint32, a workgroup adds at most
+// block * nup < 2^32 to a bin) or HBM
+template <bool LDSHIST>
+__device__ __forceinline__ void hist_add(const McsArgs &a, uint32_t *lds_hist, int32_t d,
+                                         uint64_t n)
+{
+    if (n == 0) return;
+    if (LDSHIST)
+        atomicAdd(&lds_hist[d], (uint32_t)n);
+    else
+        atomicAdd((unsigned long long *)&a.hist[d], (unsigned long long)n);
+}
+
+// OBS: 0 = D histogram only; 1 = joint H(D,M) per walker block
+// (produce2_multigpu.cu:489); 2 = exact per-D moments per walker block
+template <int R, bool FP64, int KEYING, bool LDSHIST, int OBS>
 __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
 {
     typedef typename std::conditional<FP64, double, float>::type T;
@@ -160,57 +219,125 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
         const uint64_t W = a.W;
         int32_t e = a.E[w], d = a.D[w], m = a.M[w];
         uint64_t *jbase = nullptr;
-        if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
-        uint32_t c0 = 0, pos = 0;
+        if (OBS == 1) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
+        // per-D moment arrays of this walker's block, mstride words apart
+        const uint64_t mstride = (uint64_t)a.N + 1;
+        uint64_t *mbase = nullptr;
+        if (OBS == 2) mbase = a.moms + (w / a.block_walkers) * (uint64_t)kMomWords * mstride;
+        MomRun run{};
+        run.d = -1;
+        // Histogram entries are counted in registers while the bin stays the
+        // same (D, or (D, M) for the joint histogram, changes only on some
+        // accepted moves) and added with one atomic when it changes: the
+        // integer sums are the same, the atomics far fewer.
+        uint64_t hrun = 0, jrun = 0;
+        int32_t hbin = 0;
+        int64_t jbin = 0;
+        // Attempt t+1's site, neighbours and spins are loaded before attempt t
+        // computes, so their memory latency overlaps attempt t's ln W gathers,
+        // store and histogram atomic.  That is exact when attempt t's site is
+        // neither attempt t+1's site nor one of its neighbours: for even L >= 4
+        // consecutive sites of the order 0, 2, .., N-2, 1, 3, .., N-1 differ by
+        // 2 (or are N-2 -> 1 and N-1 -> 0), never by 0, +-1, +-L, 1-L, L-1 or
+        // N-L modulo N.  Other L load after the store.
+        const bool ahead = (a.L % 2 == 0) && a.L >= 4;
+        uint32_t c0 = 0;
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
-            for (uint64_t i0 = 0; i0 < n; i0 += 4) {
-                ++c0;
-                const philox4 r1 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k1a, k1b);
-                const philox4 r3 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k3a, k3b);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (i0 + j < n) {
-                        const uint32_t idx = a.order[pos];
-                        const uint32_t *nv = a.naver + 4u * idx;
-                        const int32_t old_s = s[(uint64_t)idx * W];
-                        int32_t new_s = old_s + (r3.v[j] < 0x80000000u ? 1 : -1);
-                        new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
-                        const int32_t nb = s[(uint64_t)nv[0] * W] + s[(uint64_t)nv[1] * W] +
-                                           s[(uint64_t)nv[2] * W] + s[(uint64_t)nv[3] * W];
-                        const int32_t dE = -(new_s - old_s) * nb;
-                        const int32_t dDel = new_s * new_s - old_s * old_s;
-                        // D stays in [0, N] for a consistent (lattice, D) pair; the
-                        // clamp keeps every table access in bounds for any input
-                        const int32_t dn = min(max(d + dDel, 0), (int32_t)a.N);
-                        T diff = (T)(-(double)dE * a.beta);
-                        diff = diff + lnw_at<T>(a, dn);
-                        diff = diff - lnw_at<T>(a, d);
-                        bool acc = diff >= (T)0;
-                        if (!acc) acc = (T)acclog_u01(r1.v[j]) < diff;
-                        if (acc) {
-                            s[(uint64_t)idx * W] = (int8_t)new_s;
-                            e += dE;
-                            d = dn;
-                            m += new_s - old_s;
+            if (n == 0) continue;
+            uint32_t pos = 0;
+            // spins of the attempt about to run: site, and sum of neighbours
+            uint32_t idx = a.order[0];
+            int32_t old_s = s[(uint64_t)idx * W];
+            int32_t nsum;
+            {
+                const uint32_t *nv = a.naver + 4u * idx;
+                nsum = s[(uint64_t)nv[0] * W] + s[(uint64_t)nv[1] * W] + s[(uint64_t)nv[2] * W] +
+                       s[(uint64_t)nv[3] * W];
+            }
+            philox4 r1{}, r3{};
+            for (uint64_t i = 0; i < n; i++) {
+                if ((i & 3) == 0) {
+                    ++c0;
+                    r1 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k1a, k1b);
+                    r3 = philox4x32<R>(c0, a.seed, a.iter, MUCA_CTR3, k3a, k3b);
+                }
+                const uint32_t j = (uint32_t)(i & 3);
+                const uint32_t u1 = j == 0 ? r1.v[0] : j == 1 ? r1.v[1] : j == 2 ? r1.v[2] : r1.v[3];
+                const uint32_t u3 = j == 0 ? r3.v[0] : j == 1 ? r3.v[1] : j == 2 ? r3.v[2] : r3.v[3];
+                const bool more = i + 1 < n;
+                const uint32_t npos = (pos + 1 == a.N) ? 0 : pos + 1;
+                uint32_t nidx = 0;
+                int32_t nold = 0, nnsum = 0;
+                if (ahead && more) {
+                    nidx = a.order[npos];
+                    const uint32_t *nv = a.naver + 4u * nidx;
+                    nold = s[(uint64_t)nidx * W];
+                    nnsum = s[(uint64_t)nv[0] * W] + s[(uint64_t)nv[1] * W] +
+                            s[(uint64_t)nv[2] * W] + s[(uint64_t)nv[3] * W];
+                }
+                int32_t new_s = old_s + (u3 < 0x80000000u ? 1 : -1);
+                new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
+                const int32_t dE = -(new_s - old_s) * nsum;
+                const int32_t dDel = new_s * new_s - old_s * old_s;
+                // D stays in [0, N] for a consistent (lattice, D) pair; the
+                // clamp keeps every table access in bounds for any input
+                const int32_t dn = min(max(d + dDel, 0), (int32_t)a.N);
+                T diff = (T)(-(double)dE * a.beta);
+                diff = diff + lnw_at<T>(a, dn);
+                diff = diff - lnw_at<T>(a, d);
+                bool acc = diff >= (T)0;
+                if (!acc) acc = (T)acclog_u01(u1) < diff;
+                if (acc) {
+                    s[(uint64_t)idx * W] = (int8_t)new_s;
+                    e += dE;
+                    d = dn;
+                    m += new_s - old_s;
+                }
+                if (phase) {
+                    if (d != hbin) {
+                        hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
+                        hrun = 0;
+                        hbin = d;
+                    }
+                    hrun++;
+                    if (OBS == 1) {
+                        const int64_t mm = min(max((int64_t)m, -(int64_t)d), (int64_t)d);
+                        const int64_t jb = (mm + (int64_t)d * d + 2 * (int64_t)d) / 2;
+                        if (jb != jbin) {
+                            if (jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
+                            jrun = 0;
+                            jbin = jb;
                         }
-                        if (phase) {
-                            if (LDSHIST)
-                                atomicAdd(&lds_hist[d], 1u);
-                            else
-                                atomicAdd((unsigned long long *)&a.hist[d], 1ull);
-                            if (JOINT) {
-                                const int64_t mm = min(max((int64_t)m, -(int64_t)d), (int64_t)d);
-                                const int64_t jb = (mm + (int64_t)d * d + 2 * (int64_t)d) / 2;
-                                atomicAdd((unsigned long long *)&jbase[jb], 1ull);
-                            }
+                        jrun++;
+                    }
+                    if (OBS == 2) {
+                        if (d != run.d) {
+                            mom_flush(run, mbase, mstride);
+                            run = MomRun{};
+                            run.d = d;
                         }
-                        pos = (pos + 1 == a.N) ? 0 : pos + 1;
+                        mom_add(run, min(max(m, -d), d));
                     }
                 }
+                if (more) {
+                    if (!ahead) {
+                        nidx = a.order[npos];
+                        const uint32_t *nv = a.naver + 4u * nidx;
+                        nold = s[(uint64_t)nidx * W];
+                        nnsum = s[(uint64_t)nv[0] * W] + s[(uint64_t)nv[1] * W] +
+                                s[(uint64_t)nv[2] * W] + s[(uint64_t)nv[3] * W];
+                    }
+                    idx = nidx;
+                    old_s = nold;
+                    nsum = nnsum;
+                }
+                pos = npos;
             }
-            pos = 0;
         }
+        hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
+        if (OBS == 1 && jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
+        if (OBS == 2) mom_flush(run, mbase, mstride);
         a.E[w] = e;
         a.D[w] = d;
         a.M[w] = m;
@@ -224,6 +351,28 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
     }
 }
 
+template <int R, bool FP64, int KEYING, bool LDSHIST, int OBS>
+static void launch_generic_v(hipStream_t st, dim3 grid, int block, size_t lds, const McsArgs &a)
+{
+    auto k = k_mcs_generic<R, FP64, KEYING, LDSHIST, OBS>;
+    if (LDSHIST)
+        hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+    hipLaunchKernelGGL(k, grid, dim3(block), LDSHIST ? lds : 0, st, a);
+}
+
+template <int R, bool FP64, int KEYING, bool LDSHIST>
+static void launch_generic_h(hipStream_t st, dim3 grid, int block, size_t lds, const McsArgs &a)
+{
+    // the produce path computes in FP64, so only FP64 builds the moments variant
+    if (a.joint)
+        launch_generic_v<R, FP64, KEYING, LDSHIST, 1>(st, grid, block, lds, a);
+    else if (FP64 && a.moms)
+        launch_generic_v<R, FP64, KEYING, LDSHIST, FP64 ? 2 : 0>(st, grid, block, lds, a);
+    else
+        launch_generic_v<R, FP64, KEYING, LDSHIST, 0>(st, grid, block, lds, a);
+}
+
 template <int R, bool FP64, int KEYING>
 static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
@@ -232,27 +381,10 @@ static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsA
     const size_t lds = (size_t)(a.N + 1) * sizeof(uint32_t);
     // uint32 LDS bins cannot overflow: a workgroup adds at most block * nup
     const bool lds_ok = lds <= cfg.lds_max && (uint64_t)block * a.nup < (1ull << 32);
-    const bool joint = a.joint != nullptr;
-    if (lds_ok) {
-        if (joint) {
-            auto k = k_mcs_generic<R, FP64, KEYING, true, true>;
-            hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-            hipLaunchKernelGGL(k, grid, dim3(block), lds, st, a);
-        } else {
-            auto k = k_mcs_generic<R, FP64, KEYING, true, false>;
-            hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-            hipLaunchKernelGGL(k, grid, dim3(block), lds, st, a);
-        }
-    } else {
-        if (joint)
-            hipLaunchKernelGGL((k_mcs_generic<R, FP64, KEYING, false, true>), grid, dim3(block),
-                               0, st, a);
-        else
-            hipLaunchKernelGGL((k_mcs_generic<R, FP64, KEYING, false, false>), grid,
-                               dim3(block), 0, st, a);
-    }
+    if (lds_ok)
+        launch_generic_h<R, FP64, KEYING, true>(st, grid, block, lds, a);
+    else
+        launch_generic_h<R, FP64, KEYING, false>(st, grid, block, lds, a);
     return hipGetLastError();
 }
 
@@ -266,11 +398,14 @@ static hipError_t launch_generic_k(hipStream_t st, const Launch &cfg, const McsA
 hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
     if (a.W == 0 || (a.therm == 0 && a.nup == 0)) return hipSuccess;
-    if (cfg.kernel == 2) {
+    // per-D moment accumulation exists in the generic kernel only
+    const bool moments = a.moms != nullptr && a.joint == nullptr;
+    if (cfg.kernel == 2 && !moments) {
         TeamLayout lay{};
         if (team_layout(a, cfg.fp64, cfg.lds_max, lay)) return launch_mcs_team(st, cfg, a, lay);
     }
-    if (cfg.kernel == 3 && lane_eligible(a, cfg.fp64, cfg.lds_max)) return launch_mcs_lane(st, cfg, a);
+    if (cfg.kernel == 3 && !moments && lane_eligible(a, cfg.fp64, cfg.lds_max))
+        return launch_mcs_lane(st, cfg, a);
     if (cfg.fp64)
         return cfg.keying == 0 ? launch_generic_k<true, 0>(st, cfg, a)
                                : launch_generic_k<true, 1>(st, cfg, a);
@@ -302,6 +437,37 @@ __global__ void k_acclog_fast(uint32_t k0, uint32_t n, float *out)
 hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *out)
 {
     hipLaunchKernelGGL(k_acclog_fast, dim3((n + 255) / 256), dim3(256), 0, st, k0, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// produce path: exact per-D moments (S0, S1, limb words of S2 and S4) -> FP64
+// in the layout of k_joint_moments' output
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_moments_fp64(const uint64_t *moms, uint32_t N,
+                                                      uint32_t blocks, double *mom,
+                                                      uint32_t *range_flag)
+{
+    const uint64_t nb = (uint64_t)N + 1;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * blocks) return;
+    const uint64_t b = t / nb, D = t - b * nb;
+    const uint64_t *s = moms + b * kMomWords * nb + D;
+    double *o = mom + b * 4 * nb + D;
+    const double two32 = 4294967296.0, two64 = 18446744073709551616.0;
+    if (s[0] >> 32) atomicOr(range_flag, 1u);   // limb words may have wrapped
+    o[0] = (double)s[0];
+    o[nb] = (double)s[nb];
+    o[2 * nb] = (double)s[4 * nb] * two64 + ((double)s[3 * nb] * two32 + (double)s[2 * nb]);
+    o[3 * nb] = (double)s[7 * nb] * two64 + ((double)s[6 * nb] * two32 + (double)s[5 * nb]);
+}
+
+hipError_t launch_moments_fp64(hipStream_t st, const uint64_t *moms, uint32_t N,
+                               uint32_t blocks, double *mom, uint32_t *range_flag)
+{
+    const uint64_t n = ((uint64_t)N + 1) * blocks;
+    hipLaunchKernelGGL(k_moments_fp64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       moms, N, blocks, mom, range_flag);
     return hipGetLastError();
 }
 

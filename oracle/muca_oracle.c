@@ -219,6 +219,28 @@ void orc_state(uint32_t L, const uint32_t *naver, const int8_t *s, int32_t *E, i
     *M = m;
 }
 
+/* ---- per-D moments of the production attempts --------------------------
+ * The quantities observables() (produce2_multigpu.cu:498-518) reads off the
+ * joint histogram, kept per D instead: mom[0..5][D] (stride nb) = count,
+ * sum |M|, sum M^2 (low, high 64-bit words), sum M^4 (low, high), exact in
+ * unsigned 128-bit arithmetic.
+ * --------------------------------------------------------------------- */
+void orc_moment_add(uint64_t *mom, uint64_t nb, int32_t d, int32_t m)
+{
+    const unsigned __int128 am = (unsigned __int128)(m < 0 ? -(int64_t)m : (int64_t)m);
+    const unsigned __int128 m2 = am * am, m4 = m2 * m2;
+    unsigned __int128 s2 = ((unsigned __int128)mom[3 * nb + d] << 64) | mom[2 * nb + d];
+    unsigned __int128 s4 = ((unsigned __int128)mom[5 * nb + d] << 64) | mom[4 * nb + d];
+    s2 += m2;
+    s4 += m4;
+    mom[d] += 1;
+    mom[nb + d] += (uint64_t)am;
+    mom[2 * nb + d] = (uint64_t)s2;
+    mom[3 * nb + d] = (uint64_t)(s2 >> 64);
+    mom[4 * nb + d] = (uint64_t)s4;
+    mom[5 * nb + d] = (uint64_t)(s4 >> 64);
+}
+
 /* ------------------------------------------------------------------------
  * R8 / R18  per-walker chain (muca_delta_multigpu.cu:735-829;
  * produce2_multigpu.cu:401-496).  Both phases share the counter: ctr[0] is
@@ -227,7 +249,8 @@ void orc_state(uint32_t L, const uint32_t *naver, const int8_t *s, int32_t *E, i
  * --------------------------------------------------------------------- */
 void orc_mcs_walker(const orc_run_params *p, const uint32_t *naver, const uint32_t *order,
                     const float *lnw32, const double *lnw64, uint32_t w, int8_t *s,
-                    int32_t *E, int32_t *D, int32_t *M, uint64_t *hist, uint64_t *joint)
+                    int32_t *E, int32_t *D, int32_t *M, uint64_t *hist, uint64_t *joint,
+                    uint64_t *mom)
 {
     const uint64_t N = (uint64_t)p->L * p->L;
     uint32_t k1[2], k3[2];
@@ -285,6 +308,7 @@ void orc_mcs_walker(const orc_run_params *p, const uint32_t *naver, const uint32
             if (phase) {
                 if (hist) hist[d]++;
                 if (joint) joint[((int64_t)m + (int64_t)d * d + 2 * (int64_t)d) / 2]++;
+                if (mom) orc_moment_add(mom, N + 1, d, m);
             }
         }
     }
@@ -301,7 +325,7 @@ typedef struct {
     uint32_t w0, wa, wb;
     int8_t *lat;
     int32_t *E, *D, *M;
-    uint64_t *hist, *joint;
+    uint64_t *hist, *joint, *mom;
 } batch_arg;
 
 static void *batch_worker(void *vp)
@@ -312,14 +336,15 @@ static void *batch_worker(void *vp)
         uint32_t j = w - a->w0;
         int32_t dummyM = 0;
         orc_mcs_walker(a->p, a->naver, a->order, a->lnw32, a->lnw64, w, a->lat + (uint64_t)j * N,
-                       a->E + j, a->D + j, a->M ? a->M + j : &dummyM, a->hist, a->joint);
+                       a->E + j, a->D + j, a->M ? a->M + j : &dummyM, a->hist, a->joint,
+                       a->mom);
     }
     return NULL;
 }
 
 void orc_mcs_batch(const orc_run_params *p, const float *lnw32, const double *lnw64,
                    uint32_t w0, uint32_t nw, int8_t *lat, int32_t *E, int32_t *D,
-                   int32_t *M, uint64_t *hist, uint64_t *joint, int nthreads)
+                   int32_t *M, uint64_t *hist, uint64_t *joint, uint64_t *mom, int nthreads)
 {
     uint64_t N = (uint64_t)p->L * p->L;
     uint32_t *naver = (uint32_t *)malloc(sizeof(uint32_t) * 4 * N);
@@ -329,7 +354,8 @@ void orc_mcs_batch(const orc_run_params *p, const float *lnw32, const double *ln
     if (nthreads > 256) nthreads = 256;
     if ((uint32_t)nthreads > nw) nthreads = nw ? (int)nw : 1;
     if (nthreads == 1) {
-        batch_arg a = {p, naver, order, lnw32, lnw64, w0, w0, w0 + nw, lat, E, D, M, hist, joint};
+        batch_arg a = {p, naver, order, lnw32, lnw64, w0, w0, w0 + nw, lat, E, D, M, hist, joint,
+                       mom};
         batch_worker(&a);
     } else {
         pthread_t th[256];
@@ -339,7 +365,8 @@ void orc_mcs_batch(const orc_run_params *p, const float *lnw32, const double *ln
             hs[t] = hist ? (uint64_t *)calloc(N + 1, sizeof(uint64_t)) : NULL;
             uint32_t a0 = w0 + (uint32_t)((uint64_t)nw * t / nthreads);
             uint32_t a1 = w0 + (uint32_t)((uint64_t)nw * (t + 1) / nthreads);
-            batch_arg a = {p, naver, order, lnw32, lnw64, w0, a0, a1, lat, E, D, M, hs[t], NULL};
+            batch_arg a = {p, naver, order, lnw32, lnw64, w0, a0, a1, lat, E, D, M, hs[t], NULL,
+                           NULL};
             args[t] = a;
             pthread_create(&th[t], NULL, batch_worker, &args[t]);
         }

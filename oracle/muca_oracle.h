@@ -81,6 +81,7 @@ void orc_state(uint32_t L, const uint32_t *naver, const int8_t *s, int32_t *E, i
  * the acceptance uses (float|double)(-dE*beta), identical at beta = 2.
  * hist (nullable): N+1 uint64 bins incremented at D after each production
  * attempt.  joint (nullable): (N+1)(N+2)/2 uint64 bins at (M+D*D+2*D)/2.
+ * mom (nullable): 6 (N+1) uint64, the per-D moments of orc_moment_add.
  * lnw32 is read in ref32 mode, lnw64 in fp64 mode. */
 typedef struct {
     uint32_t L;
@@ -94,13 +95,17 @@ typedef struct {
 
 void orc_mcs_walker(const orc_run_params *p, const uint32_t *naver, const uint32_t *order,
                     const float *lnw32, const double *lnw64, uint32_t walker, int8_t *s,
-                    int32_t *E, int32_t *D, int32_t *M, uint64_t *hist, uint64_t *joint);
+                    int32_t *E, int32_t *D, int32_t *M, uint64_t *hist, uint64_t *joint,
+                    uint64_t *mom);
+/* per-D moments after one production attempt at (D, M): count, sum |M|,
+ * 128-bit sums of M^2 and M^4 as (low, high) words; six arrays of nb words */
+void orc_moment_add(uint64_t *mom, uint64_t nb, int32_t d, int32_t m);
 
 /* walkers [w0, w0+nw) with lattices at lat + (w-w0)*N; hist is shared and
- * summed; nthreads pthreads (0 = 1).  joint must be NULL when nthreads > 1. */
+ * summed; nthreads pthreads (0 = 1).  joint and mom must be NULL when nthreads > 1. */
 void orc_mcs_batch(const orc_run_params *p, const float *lnw32, const double *lnw64,
                    uint32_t w0, uint32_t nw, int8_t *lat, int32_t *E, int32_t *D,
-                   int32_t *M, uint64_t *hist, uint64_t *joint, int nthreads);
+                   int32_t *M, uint64_t *hist, uint64_t *joint, uint64_t *mom, int nthreads);
 
 /* ---- R10: hist_range (muca_delta_multigpu.cu:573-596) ---------------- */
 void orc_hist_range(const uint64_t *hist, int N, int *start, int *end);

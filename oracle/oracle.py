@@ -68,7 +68,8 @@ def lib():
         L.orc_state.argtypes = [ctypes.c_uint32, _u32p, _u8p, _i32p, _i32p, _i32p]
         L.orc_mcs_batch.argtypes = [ctypes.POINTER(RunParams), _f32p, _f64p, ctypes.c_uint32,
                                     ctypes.c_uint32, _u8p, _i32p, _i32p, _i32p, _u64p, _u64p,
-                                    ctypes.c_int]
+                                    _u64p, ctypes.c_int]
+        L.orc_moment_add.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32]
         L.orc_hist_range.argtypes = [_u64p, ctypes.c_int, _i32p, _i32p]
         L.orc_kull_leibler.restype = ctypes.c_double
         L.orc_kull_leibler.argtypes = [_u64p, ctypes.c_int]
@@ -158,11 +159,12 @@ def state(L, lat):
 
 def mcs(L, lat, E, D, M, *, keying, fp64, lnw32=None, lnw64=None, iter=0,
         seed=SEED_DEFAULT, therm=0, nupdates=0, beta=2.0, rounds=7, w0=0, hist=None,
-        joint=None, nthreads=1):
+        joint=None, moments=None, nthreads=1):
     """Run the per-walker chain for every walker of lat (in place).
 
-    Returns the D histogram (uint64, N+1) summed over walkers, and the joint
-    histogram when one is passed in."""
+    Returns the D histogram (uint64, N+1) summed over walkers; the joint
+    histogram and the per-D moments (6 (N+1) uint64, see orc_moment_add) are
+    accumulated into the arrays passed in."""
     N = L * L
     assert lat.dtype == np.int8 and lat.flags.c_contiguous and lat.shape[1] == N
     W = lat.shape[0]
@@ -177,7 +179,7 @@ def mcs(L, lat, E, D, M, *, keying, fp64, lnw32=None, lnw64=None, iter=0,
         M = np.zeros(W, np.int32)
     lib().orc_mcs_batch(ctypes.byref(p), _p(lnw32, _f32p), _p(lnw64, _f64p), w0, W,
                         _p(lat, _u8p), _p(E, _i32p), _p(D, _i32p), _p(M, _i32p),
-                        _p(hist, _u64p), _p(joint, _u64p), nthreads)
+                        _p(hist, _u64p), _p(joint, _u64p), _p(moments, _u64p), nthreads)
     return hist
 
 
@@ -239,6 +241,50 @@ def joint_moments(N, joint):
     S = [np.zeros(N + 1, np.float64) for _ in range(4)]
     lib().orc_joint_moments(N, _p(joint, _u64p), *[_p(s, _f64p) for s in S])
     return S
+
+
+def joint_moments_exact(N, joint):
+    """Per-D moments of a joint H(D,M) as exact Python integers: S0 = sum_M H,
+    S1 = sum |M| H, S2 = sum M^2 H, S4 = sum M^4 H, over the bins
+    (M + D^2 + 2D)/2 of produce2_multigpu.cu:489."""
+    S = [[0] * (N + 1) for _ in range(4)]
+    for D in range(N + 1):
+        base = (D * D + D) // 2          # M = -D
+        h = joint[base:base + D + 1]
+        for j in np.nonzero(h)[0]:
+            c = int(h[j])
+            am = abs(2 * int(j) - D)
+            S[0][D] += c
+            S[1][D] += am * c
+            S[2][D] += am * am * c
+            S[3][D] += am ** 4 * c
+    return S
+
+
+def moments_as_int(mom, N):
+    """6 (N+1) uint64 words (orc_moment_add's layout) -> S0, S1, S2, S4 as
+    arrays of Python integers."""
+    w = np.asarray(mom, np.uint64).reshape(6, N + 1).astype(object)
+    return w[0], w[1], w[2] + (w[3] << 64), w[4] + (w[5] << 64)
+
+
+def observables_moments(beta, N, S0, S1, S2, boltz_):
+    """observables() (produce2_multigpu.cu:498-518) summed per D from exact
+    integer moments, in long double: <|M|> = sum S1 b / sum S0 b and
+    chi = beta (sum S2 b / sum S0 b - <|M|>^2) / N."""
+    def ld(x):
+        x = int(x)
+        return np.longdouble(np.uint64(x >> 64)) * np.longdouble(2.0) ** 64 + \
+            np.longdouble(np.uint64(x & 0xFFFFFFFFFFFFFFFF))
+    z = m1 = m2 = np.longdouble(0)
+    for D in range(N + 1):
+        if S0[D]:
+            b = np.longdouble(boltz_[D])
+            z += ld(S0[D]) * b
+            m1 += ld(S1[D]) * b
+            m2 += ld(S2[D]) * b
+    a = m1 / z
+    return float(a), float((m2 / z - a * a) * np.longdouble(beta) / np.longdouble(N))
 
 
 def weight_iteration_reference(L, walkers, iters, *, seed=SEED_DEFAULT, init_nup=1,

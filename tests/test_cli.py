@@ -55,6 +55,12 @@ def test_produce_short_weight_file(tmp_path):
     assert p.returncode == 1 and "fewer than N+1" in p.stderr
 
 
+def test_produce_moments_excludes_host_observables(tmp_path):
+    p = run([PRODUCE, "4", "0", "0", "2", "1", "1", "1", "1.9", "0.1", f"--outdir={tmp_path}",
+             "--moments", "--host-observables"])
+    assert p.returncode == 1 and "--moments" in p.stderr
+
+
 # ---- replica of the reference weight loop (all jacks) ------------------------------
 def weight_replica(orc, L, W, jacks, init_nup, seed, recursion, lnw0=None, max_iter=0):
     N = L * L
@@ -209,8 +215,9 @@ def produce_replica(orc, L, W, B, rng_, therm_m, nup_m, d0, gap, lnw, seed=100):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("host_obs", [True, False])
-def test_produce_cli_observables(orc, tmp_path, host_obs):
+@pytest.mark.parametrize("mode", ["host", "device", "moments"])
+def test_produce_cli_observables(orc, tmp_path, mode):
+    host_obs = mode == "host"
     L, W, B, R = 4, 64, 4, 3
     N = L * L
     lnw = -0.8 * np.arange(N + 1) + 0.01 * np.arange(N + 1) ** 2
@@ -223,6 +230,8 @@ def test_produce_cli_observables(orc, tmp_path, host_obs):
             f"--jack-blocks={B}", f"--outdir={tmp_path}"]
     if host_obs:
         args.append("--host-observables")
+    if mode == "moments":
+        args.append("--moments")
     p = run(args)
     assert p.returncode == 0, p.stderr
     assert f" TOTAL UPDATES : {N * 50 * W} updates by each core : {N * 50} Therm run : {N * 20}" \

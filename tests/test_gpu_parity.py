@@ -217,6 +217,88 @@ def test_produce_joint_histograms_per_block(orc, kernel):
         assert np.array_equal(gM[b * bw:(b + 1) * bw], m)
 
 
+@pytest.mark.parametrize("L,W,B,ordered,nup_mult", [
+    (8, 96, 3, False, 7),      # random start, many D changes
+    (64, 128, 2, True, 10),    # ordered start: sum M^4 per D passes 2^64 (high words)
+    (256, 64, 1, True, 0),     # |M| = 2^16 at the start: M^4 = 2^64 exactly
+], ids=["L8-random", "L64-ordered", "L256-ordered"])
+def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult):
+    """Per-D moments accumulated by the update kernel (OBS_MOMENTS) in place
+    of the joint histogram: identical to the oracle chain's exact 128-bit
+    sums, block by block, with the final M."""
+    N = L * L
+    nup = nup_mult * N + 1 if nup_mult else 2000
+    rng = np.random.default_rng(L)
+    lnw = np.cumsum(rng.normal(size=N + 1)) if not ordered else np.zeros(N + 1)
+    if ordered:
+        lat = np.ones((W, N), np.int8)
+        E, D, Mg = orc.state(L, lat)
+    else:
+        lat, E, D, Mg = oracle_init(orc, M.KEYING_PRODUCE, L, W, seed=100)
+    with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B,
+                   observables=M.OBS_MOMENTS) as ctx:
+        assert ctx.info.kernel == M.KERNEL_GENERIC and ctx.info.joint_bins == 0
+        ctx.set_lattice(lat.T)
+        ctx.set_state(E, D, Mg)
+        ctx.set_log_weight(lnw)
+        ctx.run(1, 200, 0, nup)
+        moms = [ctx.get_moments(b) for b in range(B)]
+        _, _, gM = ctx.get_state()
+    kw = dict(keying=M.KEYING_PRODUCE, fp64=True, lnw64=lnw, nthreads=1)
+    bw = W // B
+    hi = 0
+    for b in range(B):
+        mom = np.zeros(6 * (N + 1), np.uint64)
+        sub = np.ascontiguousarray(lat[b * bw:(b + 1) * bw])
+        e, d, m = E[b * bw:(b + 1) * bw].copy(), D[b * bw:(b + 1) * bw].copy(), \
+            Mg[b * bw:(b + 1) * bw].copy()
+        orc.mcs(L, sub, e, d, m, iter=1, seed=200, therm=0, nupdates=nup, w0=b * bw,
+                moments=mom, **kw)
+        want = orc.moments_as_int(mom, N)
+        for k in range(4):
+            assert list(moms[b][k]) == list(want[k]), (b, k)
+        assert sum(moms[b][0]) == bw * nup
+        assert np.array_equal(gM[b * bw:(b + 1) * bw], m)
+        hi = max(hi, max(int(x) >> 64 for x in want[3]))
+    if ordered:
+        assert hi > 0   # the 128-bit carry path was taken
+
+
+def test_reweighting_from_moments_matches_joint(orc):
+    """The same production run accumulated both ways: device reweighting of
+    the per-D moments equals that of the joint histograms, and the oracle's
+    long-double sums over the exact moments."""
+    L, W, B = 8, 128, 4
+    N = L * L
+    lnw = -np.log(np.arange(1, N + 2, dtype=np.float64)) * 3.0
+    lnw -= lnw[0]
+    deltas = np.array([1.9, 1.95, 1.98, 2.0])
+    out = {}
+    for obs in (M.OBS_JOINT, M.OBS_MOMENTS):
+        with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B,
+                       kernel=M.KERNEL_GENERIC, observables=obs) as ctx:
+            ctx.init_lattice(seed=100)
+            ctx.compute_state()
+            ctx.set_log_weight(lnw)
+            ctx.run(0, 100, 50 * N, 0)
+            ctx.run(1, 2100, 0, 100 * N)
+            out[obs] = ctx.reweight(deltas)
+            if obs == M.OBS_MOMENTS:
+                moms = [ctx.get_moments(b) for b in range(B)]
+    for k in range(3):
+        np.testing.assert_allclose(out[M.OBS_MOMENTS][k], out[M.OBS_JOINT][k], rtol=1e-12)
+    absM, chi, _ = out[M.OBS_MOMENTS]
+    for i, dl in enumerate(deltas):
+        w = lnw.copy()
+        orc.weight_shift(2.0, dl, w)
+        bz = orc.boltz(2.0, dl, w)
+        for b in range(B):
+            a_ref, c_ref = orc.observables_moments(2.0, N, moms[b][0], moms[b][1], moms[b][2],
+                                                   bz)
+            assert absM[i, b] == pytest.approx(a_ref, rel=1e-12)
+            assert chi[i, b] == pytest.approx(c_ref, rel=1e-9)
+
+
 def test_reweighting_matches_oracle_observables(orc):
     """Device FP64 reweighting of the joint histograms vs the reference's
     long-double observables() (produce2_multigpu.cu:498-518)."""

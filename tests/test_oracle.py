@@ -341,6 +341,59 @@ def test_observables_closed_form(orc):
     assert S2[d] == (ms ** 2 * cs).sum() and S4[d] == (ms ** 4 * cs).sum()
 
 
+def test_moments_exact_closed_form(orc):
+    """joint_moments_exact on the hand-made shell of the test above."""
+    N, d = 16, 4
+    joint = np.zeros((N + 1) * (N + 2) // 2, np.uint64)
+    for m, c in ((-4, 3), (-2, 1), (0, 5), (2, 2), (4, 1)):
+        joint[(m + d * d + 2 * d) // 2] = c
+    S0, S1, S2, S4 = orc.joint_moments_exact(N, joint)
+    assert (S0[d], S1[d], S2[d], S4[d]) == (12, 22, 76, 1072)
+    assert sum(S0) == 12 and sum(S4) == 1072
+
+
+def test_moment_add_128_bit_words(orc):
+    """orc_moment_add at |M| = 2^16 (M^4 = 2^64 exactly: high word only) and
+    at 2^16 - 1 three times (the low word of sum M^4 wraps twice)."""
+    nb = 3
+    mom = np.zeros(6 * nb, np.uint64)
+    p = mom.ctypes.data_as(orc._u64p)
+    orc.lib().orc_moment_add(p, nb, 2, -65536)
+    S0, S1, S2, S4 = orc.moments_as_int(mom, nb - 1)
+    assert (S0[2], S1[2], S2[2], S4[2]) == (1, 65536, 2 ** 32, 2 ** 64)
+    assert mom[4 * nb + 2] == 0 and mom[5 * nb + 2] == 1
+    for _ in range(3):
+        orc.lib().orc_moment_add(p, nb, 1, 65535)
+    S0, S1, S2, S4 = orc.moments_as_int(mom, nb - 1)
+    assert (S0[1], S1[1], S2[1], S4[1]) == (3, 3 * 65535, 3 * 65535 ** 2, 3 * 65535 ** 4)
+    assert int(mom[5 * nb + 1]) == (3 * 65535 ** 4) >> 64 == 2
+
+
+def test_oracle_moments_equal_joint_moments(orc):
+    """The chain's per-D moments equal the moments of its own joint
+    histogram, exactly; the long-double observables agree both ways."""
+    L, W = 6, 12
+    N = L * L
+    lat = orc.init_lattice(orc.KEYING_PRODUCE, L, W, seed=7)
+    E, D, Mg = orc.state(L, lat)
+    lnw = -np.log(np.arange(1, N + 2, dtype=np.float64)) * 2.0
+    joint = np.zeros((N + 1) * (N + 2) // 2, np.uint64)
+    mom = np.zeros(6 * (N + 1), np.uint64)
+    orc.mcs(L, lat, E, D, Mg, keying=orc.KEYING_PRODUCE, fp64=True, lnw64=lnw, iter=1,
+            seed=3, therm=N, nupdates=40 * N, joint=joint, moments=mom)
+    S = orc.moments_as_int(mom, N)
+    X = orc.joint_moments_exact(N, joint)
+    for k in range(4):
+        assert list(S[k]) == list(X[k]), k
+    assert sum(S[0]) == W * 40 * N
+    w = lnw.copy()
+    orc.weight_shift(2.0, 1.95, w)
+    bz = orc.boltz(2.0, 1.95, w)
+    a0, c0 = orc.observables(2.0, N, joint, bz)
+    a1, c1 = orc.observables_moments(2.0, N, S[0], S[1], S[2], bz)
+    assert a1 == pytest.approx(a0, rel=1e-14) and c1 == pytest.approx(c0, rel=1e-11)
+
+
 @pytest.mark.slow
 def test_weight_loop_converges_to_exact_lnw(orc):
     """L=4: the multigpu_weight loop (trivial update) converges and its lnW
