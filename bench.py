@@ -37,7 +37,7 @@ hits it reports a lower bound.
 `--workload c5`: large production, L = 256 (65 536 spins), 16 384 walkers,
 T = 0.5, produce keying and FP64 acceptance, ln W from the surrogate, one
 launch of therm = N and NUPDATES = N per walker into ten per-block sets of
-exact per-D moments (count, sum |M|, sum M^2, sum M^4; 3.1 MB each) or,
+exact per-D moments (count, sum |M|, sum M^2, sum M^4; 4.2 MB each) or,
 with `--observables joint`, the reference's joint histograms H(D, M) (ten x
 17.2 GB in HBM), then FP64 reweighting on the device to <|M|>, chi and the
 Binder cumulant over a Delta grid with jackknife errors; block 0 at
@@ -483,6 +483,9 @@ def c5_workload(M, args):
             joint0 = ctx.get_joint(0)
         else:
             mom0 = ctx.get_moments(0)
+        # the same number of attempts without histogram accumulation: what
+        # the accumulation costs (runs after the observables were taken)
+        _, ms_therm = ctx.run(1, seed, therm + nup, 0)
     i0 = int(np.argmin(np.abs(deltas - delta0)))
     wsh = lnw.copy()
     O.weight_shift(beta, delta0, wsh)
@@ -525,11 +528,13 @@ def c5_workload(M, args):
                    "therm": f"{args.therm_mult} N", "nupdates": f"{args.nup_mult} N",
                    "observables": args.observables,
                    "accumulator_words_per_block": (N + 1) * (N + 2) // 2 if joint
-                   else 6 * (N + 1),
+                   else 8 * (N + 1),
                    "weights": "surrogate seed (not converged)"},
         "kernel": {1: "generic", 2: "team", 3: "lane"}.get(kernel_used, "?"),
         "block": block_used,
         "reweight_ms": rw_ms,
+        "therm_only": {"ms": ms_therm, "attempts_per_ns": attempts / (ms_therm * 1e-3) / 1e9,
+                       "note": "same attempts per walker, no histogram accumulation"},
         "observables": obs,
         "visited_D_block0": [int(np.argmax(S0 > 0)), int(N - np.argmax(S0[::-1] > 0))],
         "oracle_block0": {"delta": delta0, "abs_M_device": float(absM[i0, 0]),
