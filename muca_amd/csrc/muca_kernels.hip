@@ -216,7 +216,9 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
         const uint32_t k3a = KEYING == 0 ? 2u * wid + 50000u : wid + 40000u;
         const uint32_t k3b = KEYING == 0 ? 0x11cafeadu : 0xdecafeadu;
         int8_t *s = a.lat + w;
-        const uint64_t W = a.W;
+        // walker counts are 32-bit (muca_params.walkers): site * W is one
+        // 32 x 32 -> 64-bit multiply-add
+        const uint32_t W = (uint32_t)a.W;
         int32_t e = a.E[w], d = a.D[w], m = a.M[w];
         uint64_t *jbase = nullptr;
         if (OBS == 1) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
@@ -286,8 +288,16 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
                 T diff = (T)(-(double)dE * a.beta);
                 diff = diff + lnw_at<T>(a, dn);
                 diff = diff - lnw_at<T>(a, d);
-                bool acc = diff >= (T)0;
-                if (!acc) acc = (T)acclog_u01(u1) < diff;
+                // log(u) < diff (diff >= 0 included: log(u) < 0), screened with
+                // the hardware log: exact unless diff lies within the proven
+                // error band of la, where the correctly rounded log decides
+                const float la = acclog_fast_u01(u1);
+                const float band = acclog_fast_band(la);
+                bool acc = (T)(la + band) < diff;
+                const bool amb = !acc && !((T)(la - band) >= diff);
+                if (__builtin_amdgcn_ballot_w64(amb)) {
+                    if (amb) acc = (T)acclog_u01(u1) < diff;
+                }
                 if (acc) {
                     s[(uint64_t)idx * W] = (int8_t)new_s;
                     e += dE;
