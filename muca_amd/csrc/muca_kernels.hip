@@ -226,8 +226,11 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
         const uint64_t mstride = (uint64_t)a.N + 1;
         uint64_t *mbase = nullptr;
         if (OBS == 2) mbase = a.moms + (w / a.block_walkers) * (uint64_t)kMomWords * mstride;
-        MomRun run{};
+        // the current D's moment sums and the previous D's: a walker that
+        // steps back to the D it came from flushes nothing
+        MomRun run{}, prev{};
         run.d = -1;
+        prev.d = -1;
         // Histogram entries are counted in registers while the bin stays the
         // same (D, or (D, M) for the joint histogram, changes only on some
         // accepted moves) and added with one atomic when it changes: the
@@ -305,12 +308,16 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
                     m += new_s - old_s;
                 }
                 if (phase) {
-                    if (d != hbin) {
-                        hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
-                        hrun = 0;
-                        hbin = d;
+                    // with per-D moments the D histogram is the blocks' S0,
+                    // summed after the launch (k_hist_from_moments)
+                    if (OBS != 2) {
+                        if (d != hbin) {
+                            hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
+                            hrun = 0;
+                            hbin = d;
+                        }
+                        hrun++;
                     }
-                    hrun++;
                     if (OBS == 1) {
                         const int64_t mm = min(max((int64_t)m, -(int64_t)d), (int64_t)d);
                         const int64_t jb = (mm + (int64_t)d * d + 2 * (int64_t)d) / 2;
@@ -323,9 +330,16 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
                     }
                     if (OBS == 2) {
                         if (d != run.d) {
-                            mom_flush(run, mbase, mstride);
-                            run = MomRun{};
-                            run.d = d;
+                            if (d == prev.d) {
+                                const MomRun t = run;
+                                run = prev;
+                                prev = t;
+                            } else {
+                                mom_flush(prev, mbase, mstride);
+                                prev = run;
+                                run = MomRun{};
+                                run.d = d;
+                            }
                         }
                         mom_add(run, min(max(m, -d), d));
                     }
@@ -347,7 +361,10 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
         }
         hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
         if (OBS == 1 && jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
-        if (OBS == 2) mom_flush(run, mbase, mstride);
+        if (OBS == 2) {
+            mom_flush(run, mbase, mstride);
+            mom_flush(prev, mbase, mstride);
+        }
         a.E[w] = e;
         a.D[w] = d;
         a.M[w] = m;
@@ -383,6 +400,19 @@ static void launch_generic_h(hipStream_t st, dim3 grid, int block, size_t lds, c
         launch_generic_v<R, FP64, KEYING, LDSHIST, 0>(st, grid, block, lds, a);
 }
 
+// D histogram of a launch that kept per-D moments: hist[D] = sum over blocks
+// of S0[D] (every production attempt adds 1 to S0 of its walker's block)
+__global__ __launch_bounds__(256) void k_hist_from_moments(const uint64_t *moms, uint32_t N,
+                                                           uint32_t blocks, uint64_t *hist)
+{
+    const uint64_t nb = (uint64_t)N + 1;
+    const uint64_t D = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (D >= nb) return;
+    uint64_t h = 0;
+    for (uint32_t b = 0; b < blocks; b++) h += moms[(uint64_t)b * kMomWords * nb + D];
+    hist[D] = h;
+}
+
 template <int R, bool FP64, int KEYING>
 static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
@@ -395,6 +425,11 @@ static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsA
         launch_generic_h<R, FP64, KEYING, true>(st, grid, block, lds, a);
     else
         launch_generic_h<R, FP64, KEYING, false>(st, grid, block, lds, a);
+    if (FP64 && a.moms && !a.joint && a.nup > 0) {
+        const uint64_t nb = (uint64_t)a.N + 1;
+        hipLaunchKernelGGL(k_hist_from_moments, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                           st, (const uint64_t *)a.moms, a.N, a.joint_blocks, a.hist);
+    }
     return hipGetLastError();
 }
 

@@ -241,25 +241,28 @@ def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult):
         ctx.set_lattice(lat.T)
         ctx.set_state(E, D, Mg)
         ctx.set_log_weight(lnw)
-        ctx.run(1, 200, 0, nup)
+        ghist, _ = ctx.run(1, 200, 0, nup)
         moms = [ctx.get_moments(b) for b in range(B)]
         _, _, gM = ctx.get_state()
     kw = dict(keying=M.KEYING_PRODUCE, fp64=True, lnw64=lnw, nthreads=1)
     bw = W // B
     hi = 0
+    ohist = np.zeros(N + 1, np.uint64)
     for b in range(B):
         mom = np.zeros(6 * (N + 1), np.uint64)
         sub = np.ascontiguousarray(lat[b * bw:(b + 1) * bw])
         e, d, m = E[b * bw:(b + 1) * bw].copy(), D[b * bw:(b + 1) * bw].copy(), \
             Mg[b * bw:(b + 1) * bw].copy()
-        orc.mcs(L, sub, e, d, m, iter=1, seed=200, therm=0, nupdates=nup, w0=b * bw,
-                moments=mom, **kw)
+        ohist += orc.mcs(L, sub, e, d, m, iter=1, seed=200, therm=0, nupdates=nup, w0=b * bw,
+                         moments=mom, **kw)
         want = orc.moments_as_int(mom, N)
         for k in range(4):
             assert list(moms[b][k]) == list(want[k]), (b, k)
         assert sum(moms[b][0]) == bw * nup
         assert np.array_equal(gM[b * bw:(b + 1) * bw], m)
         hi = max(hi, max(int(x) >> 64 for x in want[3]))
+    # the launch's D histogram (the blocks' S0 summed on the device)
+    assert np.array_equal(ghist, ohist)
     if ordered:
         assert hi > 0   # the 128-bit carry path was taken
 
