@@ -235,9 +235,12 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
         // same (D, or (D, M) for the joint histogram, changes only on some
         // accepted moves) and added with one atomic when it changes: the
         // integer sums are the same, the atomics far fewer.
-        uint64_t hrun = 0, jrun = 0;
-        int32_t hbin = 0;
-        int64_t jbin = 0;
+        // The bin in use and the one before it are both kept, so a walker
+        // stepping back and forth between two bins adds nothing until it
+        // reaches a third.
+        uint64_t hrun = 0, hprun = 0, jrun = 0, jprun = 0;
+        int32_t hbin = 0, hpbin = 0;
+        int64_t jbin = 0, jpbin = 0;
         // Attempt t+1's site, neighbours and spins are loaded before attempt t
         // computes, so their memory latency overlaps attempt t's ln W gathers,
         // store and histogram atomic.  That is exact when attempt t's site is
@@ -312,8 +315,17 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
                     // summed after the launch (k_hist_from_moments)
                     if (OBS != 2) {
                         if (d != hbin) {
-                            hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
-                            hrun = 0;
+                            if (d == hpbin && hprun) {
+                                const uint64_t tr = hrun;
+                                hrun = hprun;
+                                hprun = tr;
+                                hpbin = hbin;
+                            } else {
+                                hist_add<LDSHIST>(a, lds_hist, hpbin, hprun);
+                                hprun = hrun;
+                                hpbin = hbin;
+                                hrun = 0;
+                            }
                             hbin = d;
                         }
                         hrun++;
@@ -322,8 +334,17 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
                         const int64_t mm = min(max((int64_t)m, -(int64_t)d), (int64_t)d);
                         const int64_t jb = (mm + (int64_t)d * d + 2 * (int64_t)d) / 2;
                         if (jb != jbin) {
-                            if (jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
-                            jrun = 0;
+                            if (jb == jpbin && jprun) {
+                                const uint64_t tr = jrun;
+                                jrun = jprun;
+                                jprun = tr;
+                                jpbin = jbin;
+                            } else {
+                                if (jprun) atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
+                                jprun = jrun;
+                                jpbin = jbin;
+                                jrun = 0;
+                            }
                             jbin = jb;
                         }
                         jrun++;
@@ -360,7 +381,9 @@ __global__ __launch_bounds__(256) void k_mcs_generic(McsArgs a)
             }
         }
         hist_add<LDSHIST>(a, lds_hist, hbin, hrun);
+        hist_add<LDSHIST>(a, lds_hist, hpbin, hprun);
         if (OBS == 1 && jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
+        if (OBS == 1 && jprun) atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
         if (OBS == 2) {
             mom_flush(run, mbase, mstride);
             mom_flush(prev, mbase, mstride);
