@@ -117,8 +117,10 @@ int muca_set_log_weight(muca_ctx *ctx, const double *lnw, uint32_t nbins);
  * nupdates attempts with histogram accumulation, per walker.  The D
  * histogram (nbins uint64) is zeroed first and copied to hist_out when it is
  * non-NULL; kernel_ms (nullable) receives the update kernel's duration.
- * In produce mode with joint_blocks > 0 the joint H(D,M) is zeroed and
- * accumulated too (read it with muca_get_joint). */
+ * In MUCA_FP64 mode with joint_blocks > 0 the joint H(D,M) is zeroed and
+ * accumulated too (read it with muca_get_joint), or with MUCA_OBS_MOMENTS
+ * the per-D moments (muca_get_moments; nupdates < 2^32), whose S0 summed
+ * over the blocks is then the D histogram. */
 int muca_run(muca_ctx *ctx, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t nupdates,
              uint64_t *hist_out, float *kernel_ms);
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at
