@@ -131,6 +131,11 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         int32_t d = a.D[w], e = a.E[w], m = a.M[w];
         uint64_t *jbase = nullptr;
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
+        // joint-histogram entries of this lane's attempts, counted for the bin
+        // in use and the one before it and added with one atomic when a third
+        // bin comes up (a walker stepping back and forth adds nothing)
+        uint64_t jrun = 0, jprun = 0;
+        int64_t jbin = -1, jpbin = -1;
         uint32_t c0base = 0;
         const uint32_t RB = lay.ring;
         uint32_t gen = 0, gen_slot = 0, cur_slot = 0;   // next block to make, its slot, block blk0's slot
@@ -381,8 +386,23 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                             if (JOINT) {
                                 const int64_t dd = dv[r];
                                 const int64_t mm = min(max((int64_t)mv[r], -dd), dd);
-                                atomicAdd((unsigned long long *)&jbase[(mm + dd * dd + 2 * dd) / 2],
-                                          1ull);
+                                const int64_t jb = (mm + dd * dd + 2 * dd) / 2;
+                                if (jb != jbin) {
+                                    if (jb == jpbin) {
+                                        const uint64_t t = jrun;
+                                        jrun = jprun;
+                                        jprun = t;
+                                        jpbin = jbin;
+                                    } else {
+                                        if (jprun)
+                                            atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
+                                        jprun = jrun;
+                                        jpbin = jbin;
+                                        jrun = 0;
+                                    }
+                                    jbin = jb;
+                                }
+                                jrun++;
                             }
                         }
                     }
@@ -393,6 +413,10 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 wave_sync();
             }
             c0base += (uint32_t)((n + 3) / 4);
+        }
+        if (JOINT) {
+            if (jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
+            if (jprun) atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
         }
         // dE and dM of accepted moves were summed per lane; one reduction
         // over the team's sixteen lanes per launch.  In joint mode the
