@@ -84,3 +84,16 @@ def test_bad_arguments_raise():
         M.update_recursive(np.ones(5, np.uint64), 3, 1, np.zeros(5), np.zeros(5))
     with pytest.raises(M.MucaError):
         M.jackknife(np.zeros(0))
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(observables=7), "observables must be"),
+    (dict(precision=M.REF32, observables=M.OBS_MOMENTS), "MUCA_FP64 mode only"),
+    (dict(precision=M.FP64, observables=M.OBS_MOMENTS, kernel=M.KERNEL_TEAM),
+     "generic kernel only"),
+])
+def test_moments_mode_arguments_checked_before_any_device_call(kw, msg):
+    """muca_create validates the produce-path options before it touches a
+    device, so these fail the same way with or without a GPU."""
+    with pytest.raises(M.MucaError, match=msg):
+        M.Context(16, 64, keying=M.KEYING_PRODUCE, joint_blocks=2, **kw)
