@@ -618,7 +618,10 @@ int muca_reweight(muca_ctx *c, const double *deltas, uint32_t ndelta, double *ab
     const uint64_t nb = c->N + 1;
     if (!c->mom) {
         int rc = dev_alloc(&c->mom, (uint64_t)B * 4 * nb, "moments");
-        if (!rc) rc = dev_alloc(&c->rw_flag, 1, "range flag");
+        if (rc) return rc;
+    }
+    if (!c->rw_flag) {
+        int rc = dev_alloc(&c->rw_flag, 1, "range flag");
         if (rc) return rc;
     }
     if (ndelta > c->rw_cap) {
