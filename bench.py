@@ -231,8 +231,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
-              "build (walkers are independent chains, so the path would shard by walkers "
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
+              "(walkers are independent chains, so the path would shard by walkers "
               "with one histogram all-reduce per weight iteration)", file=sys.stderr)
         sys.exit(2)
 

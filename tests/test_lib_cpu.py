@@ -97,3 +97,14 @@ def test_moments_mode_arguments_checked_before_any_device_call(kw, msg):
     device, so these fail the same way with or without a GPU."""
     with pytest.raises(M.MucaError, match=msg):
         M.Context(16, 64, keying=M.KEYING_PRODUCE, joint_blocks=2, **kw)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it says so and exits 2
+    before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2 and "one GPU" in p.stderr and p.stdout == ""
