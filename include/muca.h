@@ -131,6 +131,14 @@ int muca_run(muca_ctx *ctx, uint32_t iter, uint32_t seed, uint64_t therm, uint64
  * histogram is overwritten.  kernel/block/ms (nullable) report the choice. */
 int muca_autotune(muca_ctx *ctx, uint64_t attempts, int32_t *kernel, uint32_t *block,
                   float *ms);
+/* Test hook (no counterpart in the reference): the team and lane kernels
+ * keep u32 histogram bins in LDS and carry 2^32 into the uint64 HBM bin when
+ * one wraps (needed once a workgroup adds 2^32 counts to one bin).  With a
+ * nonzero preload those LDS bins start at `preload` and the flush adds the
+ * difference, so a test reaches the carry with a few counts per bin while
+ * the histogram muca_run returns stays the exact count.  0 (the default)
+ * is production behaviour. */
+int muca_set_test_hist_preload(muca_ctx *ctx, uint32_t preload);
 /* D2H of the last launch's D histogram (muca_delta_multigpu.cu:401-412);
  * copy_ms (nullable) receives the copy's duration */
 int muca_get_hist(muca_ctx *ctx, uint64_t *hist, uint32_t nbins, float *copy_ms);
@@ -181,24 +189,33 @@ int muca_reweight(muca_ctx *ctx, const double *deltas, uint32_t ndelta, double *
  * reference's host loop uses the host one. */
 typedef struct muca_batch muca_batch;
 
+/* Largest NUPDATES of a batch instance.  The reference keeps NUPDATES in a
+ * 32-bit unsigned (muca_delta_multigpu.cu:89,379-382); the batched engine
+ * keeps it in 64 bits, so the seeded protocol's c * N^2 updates per walker
+ * (6.4e9 at L = 128, c = 24) are taken as given.  Below 2^32 the schedule
+ * NUPDATES = (NUPDATES + 1) * 1.01 gives the reference's values. */
+#define MUCA_MAX_NUPDATES (1ull << 48)
+
 typedef struct muca_instance {
     uint32_t L;          /* even, >= 4 */
     uint32_t walkers;
     uint32_t seed;
-    uint32_t init_nup;   /* NUPDATES of the first iteration (0: 1) */
-    double beta;         /* 2.0 reproduces the reference's -dE*2 */
     int32_t recursion;   /* 0 trivial update, 1 weight_recursion */
-    uint32_t reserved;
+    uint64_t init_nup;   /* NUPDATES of the first iteration (0: 1); larger than
+                            MUCA_MAX_NUPDATES: MUCA_E_RANGE from muca_batch_create */
+    double beta;         /* 2.0 reproduces the reference's -dE*2 */
 } muca_instance;
 
 typedef struct muca_instance_status {
     uint32_t iter;       /* iterations done */
-    uint32_t nup;        /* NUPDATES of the next iteration */
     uint32_t width;      /* widest histogram range seen (:432-433) */
     uint32_t converged;  /* width == N and Kullback-Leibler < 1e-4 (:447) */
     uint32_t reach_iter; /* iteration at which width reached N (0xFFFFFFFF: not yet) */
     uint32_t done_iter;  /* iteration that converged (0xFFFFFFFF: not yet) */
     int32_t start, end;  /* hist_range of the last iteration */
+    uint32_t nup_capped; /* 1 once the schedule would have passed MUCA_MAX_NUPDATES
+                            (NUPDATES then stays at that maximum) */
+    uint64_t nup;        /* NUPDATES of the next iteration */
     double dkl;          /* Kullback-Leibler divergence of the last iteration */
     uint64_t attempts;   /* flip attempts done (therm + nupdates per walker) */
 } muca_instance_status;
@@ -222,6 +239,13 @@ int muca_batch_get_log_weight(muca_batch *batch, uint32_t instance, double *lnw,
                               uint32_t nbins);
 /* the D histogram of the instance's last iteration */
 int muca_batch_get_hist(muca_batch *batch, uint32_t instance, uint64_t *hist, uint32_t nbins);
+/* the instance's walkers in the reference's layout s[site * walkers + walker]
+ * (bytes = N * walkers) and their E, D, M (walkers each; NULL skips one) --
+ * what muca_get_lattice / muca_get_state give for one context */
+int muca_batch_get_lattice(muca_batch *batch, uint32_t instance, int8_t *site_major,
+                           uint64_t bytes);
+int muca_batch_get_state(muca_batch *batch, uint32_t instance, int32_t *E, int32_t *D,
+                         int32_t *M);
 /* `iterations` weight iterations of every unconverged instance; use_graph
  * replays one captured hipGraph per iteration; gpu_ms (nullable) receives
  * the elapsed device time */
@@ -236,7 +260,10 @@ int muca_batch_iterate_profiled(muca_batch *batch, uint32_t iterations, float *u
  * 256 threads per block, muca_delta_multigpu.cu:27): times one iteration of
  * every eligible tile kind and workgroup size from freshly initialised walkers
  * (muca_batch_init(mm)), keeps the fastest and leaves the batch initialised
- * with mm.  kind/block/ms (nullable) report the choice. */
+ * with mm, except that every instance's ln W is what it was before the call
+ * (so weights set with muca_batch_set_log_weight survive; call order create,
+ * init, set_log_weight, autotune, iterate).  kind/block/ms (nullable) report
+ * the choice. */
 int muca_batch_autotune(muca_batch *batch, uint32_t mm, int32_t *kind, uint32_t *block,
                         float *ms);
 /* (the autotuner re-initialises with the start of the last init call) */

@@ -54,19 +54,40 @@ class Info(ctypes.Structure):
                 ("lds_lattice", ctypes.c_uint32)]
 
 
+MAX_NUPDATES = 1 << 48           # include/muca.h MUCA_MAX_NUPDATES
+
+
 class Instance(ctypes.Structure):
     _fields_ = [("L", ctypes.c_uint32), ("walkers", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("init_nup", ctypes.c_uint32),
-                ("beta", ctypes.c_double), ("recursion", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("recursion", ctypes.c_int32),
+                ("init_nup", ctypes.c_uint64), ("beta", ctypes.c_double)]
 
 
 class InstanceStatus(ctypes.Structure):
-    _fields_ = [("iter", ctypes.c_uint32), ("nup", ctypes.c_uint32),
-                ("width", ctypes.c_uint32), ("converged", ctypes.c_uint32),
-                ("reach_iter", ctypes.c_uint32), ("done_iter", ctypes.c_uint32),
-                ("start", ctypes.c_int32), ("end", ctypes.c_int32),
-                ("dkl", ctypes.c_double), ("attempts", ctypes.c_uint64)]
+    _fields_ = [("iter", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("converged", ctypes.c_uint32), ("reach_iter", ctypes.c_uint32),
+                ("done_iter", ctypes.c_uint32), ("start", ctypes.c_int32),
+                ("end", ctypes.c_int32), ("nup_capped", ctypes.c_uint32),
+                ("nup", ctypes.c_uint64), ("dkl", ctypes.c_double),
+                ("attempts", ctypes.c_uint64)]
+
+
+def pack_instances(instances):
+    """ctypes array of muca_instance; every field is range-checked first, since
+    ctypes would wrap an out-of-range integer silently."""
+    ins = list(instances)
+    arr = (Instance * len(ins))()
+    for i, q in enumerate(ins):
+        vals = {"L": q["L"], "walkers": q["walkers"], "seed": q["seed"],
+                "init_nup": q.get("init_nup", 1)}
+        for k, v in vals.items():
+            hi = MAX_NUPDATES if k == "init_nup" else 0xFFFFFFFF
+            if int(v) != v or not 0 <= v <= hi:
+                raise MucaError(f"instance {i}: {k} = {v} is outside [0, {hi}]")
+        arr[i] = Instance(int(q["L"]), int(q["walkers"]), int(q["seed"]),
+                          1 if q.get("recursion") else 0, int(vals["init_nup"]),
+                          float(q.get("beta", 2.0)))
+    return arr
 
 
 _vp = ctypes.c_void_p
@@ -89,6 +110,8 @@ _SIGS = {
     "muca_batch_set_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
     "muca_batch_get_log_weight": (ctypes.c_int, [_vp, _u32, _f64p, _u32]),
     "muca_batch_get_hist": (ctypes.c_int, [_vp, _u32, _u64p, _u32]),
+    "muca_batch_get_lattice": (ctypes.c_int, [_vp, _u32, _u8p, ctypes.c_uint64]),
+    "muca_batch_get_state": (ctypes.c_int, [_vp, _u32, _i32p, _i32p, _i32p]),
     "muca_batch_iterate": (ctypes.c_int, [_vp, _u32, ctypes.c_int32, _f32p]),
     "muca_batch_status": (ctypes.c_int, [_vp, ctypes.POINTER(InstanceStatus), _u32]),
     "muca_mlp_last_error": (ctypes.c_char_p, []),
@@ -118,6 +141,7 @@ _SIGS = {
     "muca_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint64, ctypes.c_uint64, _u64p, _f32p]),
     "muca_get_hist": (ctypes.c_int, [ctypes.c_void_p, _u64p, ctypes.c_uint32, _f32p]),
+    "muca_set_test_hist_preload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "muca_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _i32p,
                                      ctypes.POINTER(ctypes.c_uint32), _f32p]),
     "muca_get_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _u64p,
@@ -250,6 +274,11 @@ class Context:
                               ctypes.byref(ms)))
         return h, ms.value
 
+    def set_test_hist_preload(self, preload):
+        """Test hook: LDS histogram bins of the team and lane kernels start at
+        `preload` (so the u32 wrap carry runs with few counts)."""
+        _check(lib().muca_set_test_hist_preload(self._h, preload))
+
     def autotune(self, attempts):
         """Time every eligible (kernel, workgroup) on a scratch copy of the
         walkers and keep the fastest; returns (kernel, block, ms)."""
@@ -294,19 +323,17 @@ class Batch:
     muca_delta_multigpu.cu:314-476); the loop body runs on the device.
 
     instances: iterable of dicts with keys L, walkers, seed, and optionally
-    beta (2.0), init_nup (1), recursion (False)."""
+    beta (2.0), init_nup (1, at most MAX_NUPDATES), recursion (False)."""
 
     def __init__(self, instances, device=0, *, precision=REF32, tiles=KERNEL_AUTO):
         ins = list(instances)
-        arr = (Instance * len(ins))()
-        for i, q in enumerate(ins):
-            arr[i] = Instance(q["L"], q["walkers"], q["seed"], q.get("init_nup", 1),
-                              q.get("beta", 2.0), 1 if q.get("recursion") else 0, 0)
+        arr = pack_instances(ins)
         self._h = ctypes.c_void_p()
         _bcheck(lib().muca_batch_create(device, arr, len(ins), precision, tiles,
                                         ctypes.byref(self._h)))
         self.n = len(ins)
         self.N = [q["L"] * q["L"] for q in ins]
+        self.W = [q["walkers"] for q in ins]
         self._query()
 
     def _query(self):
@@ -351,6 +378,18 @@ class Batch:
         _bcheck(lib().muca_batch_get_hist(self._h, i, _p(a, _u64p), a.size))
         return a
 
+    def get_lattice(self, i):
+        """Site-major lattice of instance i, shape (N, walkers)."""
+        a = np.empty((self.N[i], self.W[i]), np.int8)
+        _bcheck(lib().muca_batch_get_lattice(self._h, i, _p(a, _u8p), a.size))
+        return a
+
+    def get_state(self, i):
+        E, D, Mg = (np.empty(self.W[i], np.int32) for _ in range(3))
+        _bcheck(lib().muca_batch_get_state(self._h, i, _p(E, _i32p), _p(D, _i32p),
+                                           _p(Mg, _i32p)))
+        return E, D, Mg
+
     def iterate(self, iterations=1, use_graph=True):
         """Returns the elapsed device milliseconds."""
         ms = ctypes.c_float()
@@ -368,7 +407,8 @@ class Batch:
 
     def autotune(self, mm=0):
         """Pick the fastest tile kind / workgroup size; leaves the batch
-        initialised with mm.  Returns (kind, block, ms of one iteration)."""
+        initialised with mm, keeping each instance's ln W as it was before the
+        call.  Returns (kind, block, ms of one iteration)."""
         k, bl, ms = ctypes.c_int32(), _u32(), ctypes.c_float()
         _bcheck(lib().muca_batch_autotune(self._h, mm, ctypes.byref(k), ctypes.byref(bl),
                                           ctypes.byref(ms)))

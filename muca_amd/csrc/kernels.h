@@ -27,6 +27,10 @@ struct McsArgs {
     uint64_t joint_stride;   // (N+1)(N+2)/2
     uint64_t block_walkers;  // walkers per joint block
     uint32_t joint_blocks;
+    // test hook: the team and lane kernels start their u32 LDS histogram bins
+    // at this value (0 in production) and flush the difference, so that a
+    // test reaches the bin-wrap carry with few counts
+    uint32_t hist_preload;
 };
 
 struct Launch {
@@ -82,8 +86,10 @@ struct BatchInst {
     uint32_t lane_hist;    // lane tiles: histogram in LDS
     TeamLayout lay;        // team tiles
     // advanced on the device
-    uint32_t iter, nup, width, reach, converged, reach_iter, done_iter;
+    uint32_t iter, width, reach, converged, reach_iter, done_iter;
     int32_t start, end;
+    uint32_t nup_capped;   // the schedule reached MUCA_MAX_NUPDATES
+    uint64_t nup;          // NUPDATES of the current iteration
     double dkl;
     uint64_t attempts;     // flip attempts done so far
 };
@@ -163,7 +169,10 @@ hipError_t launch_moments_fp64(hipStream_t st, const uint64_t *moms, uint32_t N,
 // joint H(D,M) -> per-D moments (S0, S1, S2, S4) for each block, FP64
 hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t N,
                                 uint64_t stride, uint32_t blocks, double *mom);
-// reweighting of per-D moments at ndelta crystal fields
+// p[0..n) = v (a kernel, so that callers may capture it in a graph)
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+// reweighting of per-D moments at ndelta crystal fields; out holds three
+// planes of ndelta*blocks values (<|M|>, chi, Binder), [delta][block] each
 hipError_t launch_reweight(hipStream_t st, const double *mom, const double *lnw, uint32_t N,
                            uint32_t blocks, double beta, const double *deltas,
                            uint32_t ndelta, double *out);

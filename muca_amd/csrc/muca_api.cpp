@@ -11,49 +11,11 @@
 #include <vector>
 
 #include "acclog.h"
+#include "ctx.h"
 #include "kernels.h"
 
-namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-    g_err = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(MUCA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-}  // namespace
-
-struct muca_ctx {
-    muca_params p{};
-    uint32_t N = 0;
-    uint64_t W = 0;
-    hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int8_t *lat = nullptr;
-    int32_t *E = nullptr, *D = nullptr, *M = nullptr;
-    float *lnw32 = nullptr;
-    double *lnw64 = nullptr;
-    uint32_t *order = nullptr, *naver = nullptr;
-    uint64_t *hist = nullptr;
-    uint64_t *joint = nullptr;
-    uint64_t *moms = nullptr;   // MUCA_OBS_MOMENTS: blocks * kMomWords * (N+1)
-    uint64_t joint_stride = 0, block_walkers = 0;
-    double *mom = nullptr, *deltas = nullptr, *rw_out = nullptr;
-    uint32_t *rw_flag = nullptr;
-    uint32_t rw_cap = 0;
-    bool have_weights = false;
-    muca::Launch cfg{};
-    uint32_t cus = 0;
-};
+// the thread's last error message (muca_last_error)
+thread_local std::string g_muca_err;
 
 namespace {
 
@@ -113,7 +75,7 @@ extern "C" {
 
 const char *muca_last_error(void)
 {
-    return g_err.c_str();
+    return g_muca_err.c_str();
 }
 
 int muca_default_walkers(int32_t device, uint32_t *walkers)
@@ -441,6 +403,7 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     a.joint_stride = c->joint_stride;
     a.block_walkers = c->block_walkers ? c->block_walkers : c->W;
     a.joint_blocks = c->p.joint_blocks;
+    a.hist_preload = c->hist_preload;
     HIP_TRY(hipEventRecord(c->ev0, c->st));
     HIP_TRY(muca::launch_mcs(c->st, c->cfg, a));
     HIP_TRY(hipEventRecord(c->ev1, c->st));
@@ -448,6 +411,13 @@ int muca_run(muca_ctx *c, uint32_t iter, uint32_t seed, uint64_t therm, uint64_t
     if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, c->ev0, c->ev1));
     if (hist_out)
         HIP_TRY(hipMemcpy(hist_out, c->hist, (N + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_set_test_hist_preload(muca_ctx *c, uint32_t preload)
+{
+    if (!c) return fail(MUCA_E_ARG, "NULL context");
+    c->hist_preload = preload;
     return MUCA_OK;
 }
 
@@ -651,10 +621,11 @@ int muca_reweight(muca_ctx *c, const double *deltas, uint32_t ndelta, double *ab
         HIP_TRY(hipMemcpyAsync(&range, c->rw_flag, 4, hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
     if (range) return fail(MUCA_E_RANGE, "a D bin holds 2^32 or more entries");
-    for (size_t i = 0; i < (size_t)ndelta * B; i++) {
-        if (absM) absM[i] = o[3 * i];
-        if (chi) chi[i] = o[3 * i + 1];
-        if (binder) binder[i] = o[3 * i + 2];
+    const size_t plane = (size_t)ndelta * B;
+    for (size_t i = 0; i < plane; i++) {
+        if (absM) absM[i] = o[i];
+        if (chi) chi[i] = o[plane + i];
+        if (binder) binder[i] = o[2 * plane + i];
     }
     return MUCA_OK;
 }

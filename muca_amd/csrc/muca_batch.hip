@@ -208,8 +208,16 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
             }
             if (threadIdx.x == 0) {
                 if (I->width == N) {                                 // :379-382, next iteration
-                    const uint32_t nu = I->nup + 1;
-                    I->nup = (uint32_t)(nu * 1.01);
+                    // 64-bit NUPDATES: equal to the reference's unsigned
+                    // arithmetic below 2^32, capped (never 0) above
+                    const uint64_t nu = I->nup + 1;
+                    const double grown = (double)nu * 1.01;
+                    if (grown >= (double)MUCA_MAX_NUPDATES) {
+                        I->nup = MUCA_MAX_NUPDATES;
+                        I->nup_capped = 1;
+                    } else {
+                        I->nup = (uint64_t)grown;
+                    }
                 }
                 I->iter += 1;
             }
@@ -261,7 +269,7 @@ struct muca_batch {
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<muca::BatchInst> inst;
-    std::vector<uint32_t> init_nup;
+    std::vector<uint64_t> init_nup;
     std::vector<muca::BatchTile> tiles;
     muca::BatchArgs a{};
     muca::BatchTile *d_tiles = nullptr;
@@ -361,6 +369,16 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     *out = nullptr;
     if (precision != MUCA_REF32 && precision != MUCA_FP64)
         return bfail(MUCA_E_ARG, "precision must be MUCA_REF32 or MUCA_FP64");
+    // instance checks come before any device call
+    for (uint32_t i = 0; i < n; i++) {
+        const muca_instance &q = ins[i];
+        if (q.L < 4 || q.L % 2 != 0 || q.L > 46340 || q.walkers == 0 || !(q.beta > 0))
+            return bfail(MUCA_E_ARG, "instance " + std::to_string(i) +
+                                         ": need even L >= 4, walkers > 0, beta > 0");
+        if (q.init_nup > MUCA_MAX_NUPDATES)
+            return bfail(MUCA_E_RANGE, "instance " + std::to_string(i) +
+                                           ": init_nup exceeds MUCA_MAX_NUPDATES (2^48)");
+    }
     int ndev = 0;
     BHIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return bfail(MUCA_E_ARG, "device not present");
@@ -377,11 +395,6 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     uint64_t lat = 0, st = 0, wo = 0;
     for (uint32_t i = 0; i < n; i++) {
         const muca_instance &q = ins[i];
-        if (q.L < 4 || q.L % 2 != 0 || q.L > 46340 || q.walkers == 0 || !(q.beta > 0)) {
-            batch_free(b);
-            return bfail(MUCA_E_ARG, "instance " + std::to_string(i) +
-                                         ": need even L >= 4, walkers > 0, beta > 0");
-        }
         muca::BatchInst I{};
         I.L = q.L;
         I.N = q.L * q.L;
@@ -533,7 +546,7 @@ int batch_init(muca_batch *b, uint32_t mm, int stratified)
         I.iter = 0;
         I.nup = b->init_nup[i];
         I.width = 10;   // :314
-        I.reach = I.converged = I.reach_iter = I.done_iter = 0;
+        I.reach = I.converged = I.reach_iter = I.done_iter = I.nup_capped = 0;
         I.start = I.end = 0;
         I.dkl = 0;
         I.attempts = 0;
@@ -589,6 +602,30 @@ int muca_batch_get_hist(muca_batch *b, uint32_t i, uint64_t *hist, uint32_t nbin
     BHIP(hipSetDevice(b->device));
     BHIP(hipStreamSynchronize(b->st));
     BHIP(hipMemcpy(hist, b->a.hist_last + I.w_off, nbins * 8, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_batch_get_lattice(muca_batch *b, uint32_t i, int8_t *site_major, uint64_t bytes)
+{
+    if (!b || !site_major || i >= b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    const muca::BatchInst &I = b->inst[i];
+    if (bytes != (uint64_t)I.N * I.W) return bfail(MUCA_E_ARG, "bytes must be N * walkers");
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(site_major, b->a.lat + I.lat_off, bytes, hipMemcpyDeviceToHost));
+    return MUCA_OK;
+}
+
+int muca_batch_get_state(muca_batch *b, uint32_t i, int32_t *E, int32_t *D, int32_t *M)
+{
+    if (!b || i >= b->inst.size()) return bfail(MUCA_E_ARG, "bad arguments");
+    const muca::BatchInst &I = b->inst[i];
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st));
+    int32_t *dst[3] = {E, D, M};
+    int32_t *src[3] = {b->a.E + I.st_off, b->a.D + I.st_off, b->a.M + I.st_off};
+    for (int k = 0; k < 3; k++)
+        if (dst[k]) BHIP(hipMemcpy(dst[k], src[k], (size_t)I.W * 4, hipMemcpyDeviceToHost));
     return MUCA_OK;
 }
 
@@ -672,6 +709,13 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
     if (b->team_ok) cands.push_back({MUCA_KERNEL_TEAM, 256});
     if (b->lane_ok)
         for (uint32_t blk : {256u, 512u, 1024u}) cands.push_back({MUCA_KERNEL_LANE, blk});
+    // ln W set before the call (a surrogate seed, say) survives the tuning:
+    // each candidate re-initialises the batch, which zeroes ln W
+    BHIP(hipStreamSynchronize(b->st));
+    std::vector<double> keep64(b->bins);
+    std::vector<float> keep32(b->bins);
+    BHIP(hipMemcpy(keep64.data(), b->a.lnw64, b->bins * 8, hipMemcpyDeviceToHost));
+    BHIP(hipMemcpy(keep32.data(), b->a.lnw32, b->bins * 4, hipMemcpyDeviceToHost));
     float best = 0;
     Cand pick{b->kind, b->block};
     for (const Cand &c : cands) {
@@ -689,6 +733,8 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
     BHIP(configure(b, pick.kind, pick.block));
     const int rc = batch_init(b, mm, b->stratified);
     if (rc) return rc;
+    BHIP(hipMemcpy(b->a.lnw64, keep64.data(), b->bins * 8, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->a.lnw32, keep32.data(), b->bins * 4, hipMemcpyHostToDevice));
     if (kind) *kind = pick.kind;
     if (block) *block = b->block;
     if (ms) *ms = best;
@@ -706,6 +752,7 @@ int muca_batch_status(muca_batch *b, muca_instance_status *out, uint32_t n)
         const muca::BatchInst &I = cur[i];
         out[i].iter = I.iter;
         out[i].nup = I.nup;
+        out[i].nup_capped = I.nup_capped;
         out[i].width = I.width;
         out[i].converged = I.converged;
         out[i].reach_iter = I.reach ? I.reach_iter : 0xFFFFFFFFu;

@@ -594,7 +594,8 @@ hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t 
 // reweighting (produce2_multigpu.cu:295-297,391-400,498-518), one workgroup
 // per (delta, block): shift = max_D(-lnW_D - beta*delta*D), then
 // Z = sum_D exp(-beta*delta*D - lnW_D - shift) S0_D, A_k likewise with S_k.
-// out[(id*blocks + b)*3 + 0..2] = <|M|>, chi, Binder cumulant.
+// Three planes of ndelta*blocks values: out[k*plane + id*blocks + b] with
+// k = 0, 1, 2 for <|M|>, chi, Binder cumulant.
 // ---------------------------------------------------------------------------
 __device__ static double block_reduce_sum(double v, double *sh)
 {
@@ -659,11 +660,24 @@ __global__ __launch_bounds__(256) void k_reweight(const double *mom, const doubl
     a4 = block_reduce_sum(a4, sh);
     if (threadIdx.x == 0) {
         const double m1 = a1 / z, m2 = a2 / z, m4 = a4 / z;
-        double *o = out + ((uint64_t)id * blocks + b) * 3;
+        const uint64_t plane = (uint64_t)gridDim.x;
+        double *o = out + (uint64_t)id * blocks + b;
         o[0] = m1;
-        o[1] = (m2 - m1 * m1) * beta / (double)N;
-        o[2] = 1.0 - m4 / (3.0 * m2 * m2);
+        o[plane] = (m2 - m1 * m1) * beta / (double)N;
+        o[2 * plane] = 1.0 - m4 / (3.0 * m2 * m2);
     }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_reweight(hipStream_t st, const double *mom, const double *lnw, uint32_t N,

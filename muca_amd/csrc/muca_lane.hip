@@ -34,7 +34,7 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
     const uint32_t N = a.N, L = a.L, Rw = L / 2;
     for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
         wt[i] = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
-        if (LDSHIST) hist[i] = 0;
+        if (LDSHIST) hist[i] = a.hist_preload;
     }
     __syncthreads();
     const uint64_t w = wbase + threadIdx.x;
@@ -61,7 +61,7 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
         };
         uint32_t c0 = 0;
         // a workgroup adds at most blockDim NUPDATES counts to one u32 bin
-        const bool may_wrap = (uint64_t)blockDim.x * a.nup >= (1ull << 32);
+        const bool may_wrap = a.hist_preload != 0 || (uint64_t)blockDim.x * a.nup >= (1ull << 32);
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             if (n == 0) continue;
@@ -142,9 +142,12 @@ __device__ __forceinline__ void lane_run(const McsArgs &a, uint32_t lds_w, uint6
     }
     if (LDSHIST) {
         __syncthreads();
+        // counts since the start (modulo 2^32; a wrap was carried already)
         for (uint32_t i = threadIdx.x; i <= N; i += blockDim.x) {
             const uint32_t c = hist[i];
-            if (c) atomicAdd((unsigned long long *)&a.hist[i], (unsigned long long)c);
+            if (c != a.hist_preload)
+                atomicAdd((unsigned long long *)&a.hist[i],
+                          (unsigned long long)((int64_t)c - (int64_t)a.hist_preload));
         }
     }
 }

@@ -108,7 +108,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         wt[i] = v;
     }
     if (LDSHIST)
-        for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = 0;
+        for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = a.hist_preload;
     acclog_table_to_lds(logtab, tid, blockDim.x);
     if (LATLDS) {
         for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
@@ -141,7 +141,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         uint32_t gen = 0, gen_slot = 0, cur_slot = 0;   // next block to make, its slot, block blk0's slot
         int32_t de_lane = 0, dm_lane = 0;
         // a workgroup adds at most 16 NUPDATES counts to one u32 bin
-        const bool may_wrap = (uint64_t)kTeams * a.nup >= (1ull << 32);
+        const bool may_wrap = a.hist_preload != 0 || (uint64_t)kTeams * a.nup >= (1ull << 32);
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             for (uint64_t rs = 0; rs < n; rs += Rw) {
@@ -432,9 +432,12 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     }
     __syncthreads();
     if (LDSHIST) {
+        // counts since the start (modulo 2^32; a wrap was carried already)
         for (uint32_t i = tid; i <= N; i += blockDim.x) {
             const uint32_t c = hist[i];
-            if (c) atomicAdd((unsigned long long *)&a.hist[i], (unsigned long long)c);
+            if (c != a.hist_preload)
+                atomicAdd((unsigned long long *)&a.hist[i],
+                          (unsigned long long)((int64_t)c - (int64_t)a.hist_preload));
         }
     }
     if (LATLDS) {

@@ -209,3 +209,30 @@ def test_batch_rejects_bad_instances():
         M.Batch([dict(L=7, walkers=4, seed=1)])
     with pytest.raises(M.MucaError):
         M.Batch([dict(L=12, walkers=4, seed=1)], tiles=M.KERNEL_TEAM)
+
+
+def test_batch_autotune_keeps_weights_set_before():
+    """Weights set with set_log_weight (a surrogate seed) survive the
+    autotuner, which re-initialises the walkers for every candidate."""
+    ins = _instances(TEAM_SET)
+    seeds = []
+    with M.Batch(ins) as b:
+        b.init(1)
+        for i, q in enumerate(ins):
+            N = q["L"] ** 2
+            lnw = -0.7 * np.arange(N + 1) + np.sin(0.3 * np.arange(N + 1))
+            seeds.append(lnw - lnw[0])
+            b.set_log_weight(i, seeds[-1])
+        b.autotune(1)
+        for i in range(b.n):
+            assert np.array_equal(b.get_log_weight(i), seeds[i])
+        b.iterate(1)
+        kind = b.tiles
+        got = [b.get_hist(i) for i in range(b.n)]
+    with M.Batch(ins, tiles=kind) as b:
+        b.init(1)
+        for i in range(b.n):
+            b.set_log_weight(i, seeds[i])
+        b.iterate(1)
+        for i in range(b.n):
+            assert np.array_equal(b.get_hist(i), got[i])

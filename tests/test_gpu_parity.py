@@ -330,3 +330,36 @@ def test_reweighting_matches_oracle_observables(orc):
             z = (bz * S0).sum()
             m2, m4 = (bz * S2).sum() / z, (bz * S4).sum() / z
             assert binder[i, b] == pytest.approx(1 - m4 / (3 * m2 * m2), rel=1e-10)
+
+
+@pytest.mark.parametrize("kernel,L,W,prec,nup", [
+    (M.KERNEL_TEAM, 16, 40, M.REF32, 600),
+    (M.KERNEL_TEAM, 72, 48, M.REF32, 700),
+    (M.KERNEL_TEAM, 24, 20, M.FP64, 500),
+    (M.KERNEL_LANE, 16, 300, M.REF32, 400),
+    (M.KERNEL_LANE, 12, 70, M.FP64, 300),
+], ids=["team-L16", "team-L72", "team-L24-f64", "lane-L16", "lane-L12-f64"])
+def test_lds_histogram_wrap_carry(orc, kernel, L, W, prec, nup):
+    """The u32 LDS histogram bins start 16 short of 2^32 (test hook), so every
+    bin a workgroup fills 16 or more times wraps and carries 2^32 into its
+    uint64 HBM bin; the returned histogram must still be the oracle's exact
+    count."""
+    N = L * L
+    rng = np.random.default_rng(N + W)
+    lnw = -1.2 * np.arange(N + 1) + np.cumsum(rng.normal(scale=0.8, size=N + 1))
+    lnw -= lnw[0]
+    lat, E, D, Mg = oracle_init(orc, M.KEYING_WEIGHT, L, W, seed=21)
+    with M.Context(L, W, precision=prec, kernel=kernel) as ctx:
+        ctx.set_test_hist_preload(0xFFFFFFF0)
+        ctx.set_lattice(lat.T)
+        ctx.set_state(E, D, Mg)
+        ctx.set_log_weight(lnw)
+        hist, _ = ctx.run(1, 99, N, nup)
+        glat = ctx.get_lattice()
+    h = orc.mcs(L, lat, E, D, Mg, keying=M.KEYING_WEIGHT, fp64=prec == M.FP64,
+                lnw32=lnw.astype(np.float32), lnw64=lnw, iter=1, seed=99, therm=N,
+                nupdates=nup, beta=2.0, nthreads=NTHREADS)
+    assert np.array_equal(glat, lat.T)
+    assert np.array_equal(hist, h)
+    # bins filled 16 or more times by one workgroup took the carry
+    assert int(h.max()) >= 16 * 4

@@ -108,3 +108,36 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert p.returncode == 2 and "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_batch_nupdates_is_64_bit():
+    """The seeded protocol's NUPDATES at L = 128 (24 N^2 = 6.4e9, 16 N^2 =
+    2^32) reach the library unchanged instead of wrapping to 32 bits."""
+    from muca_amd import surrogate as S
+    for factor in (16, 24):
+        nup = S.equilibrium_nupdates(128, factor)
+        arr = M.pack_instances([dict(L=128, walkers=4096, seed=1, init_nup=nup)])
+        assert arr[0].init_nup == nup and nup >= 1 << 32
+
+
+@pytest.mark.parametrize("field,value", [("init_nup", M.MAX_NUPDATES + 1), ("init_nup", -1),
+                                         ("walkers", 1 << 32), ("seed", -5), ("L", 2.5)])
+def test_batch_rejects_out_of_range_fields_before_packing(field, value):
+    q = dict(L=16, walkers=16, seed=1, init_nup=1)
+    q[field] = value
+    with pytest.raises(M.MucaError):
+        M.pack_instances([q])
+
+
+def test_batch_create_rejects_nupdates_beyond_max_before_any_device_call():
+    """muca_batch_create checks every instance before it touches a device, so
+    this runs (and fails with MUCA_E_RANGE) without a GPU."""
+    import ctypes
+    arr = (M.Instance * 1)()
+    arr[0] = M.Instance(16, 16, 1, 0, M.MAX_NUPDATES + 1, 2.0)
+    h = ctypes.c_void_p()
+    rc = M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h))
+    assert rc == -6 and not h.value
+    assert b"MUCA_MAX_NUPDATES" in M.lib().muca_batch_last_error()
+    arr[0] = M.Instance(7, 16, 1, 0, 1, 2.0)
+    assert M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h)) == -1
