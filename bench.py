@@ -20,19 +20,27 @@ instances that share (L, T) differ by seed.)
 Prints one JSON line.  `value` is whole-step throughput (attempts / wall time
 of the timed steps); `roofline` prices the update kernel alone (HIP events on
 the batch's stream, over as many further iterations launched eagerly) against
-HBM at 2 algorithmic bytes per attempt; `host_loop` is the same workload
-driven the reference's way (host weight update between launches);
-`cpu_baseline` times the CPU oracle (the same chain, threaded over walkers) on
-a bounded sample of the same workload.
+HBM at 2 algorithmic bytes per attempt; `roofline_valu` prices the same
+kernel rate against the binding ceiling, the Philox4x32-7 block rate measured
+in this run by muca_amd/libmuca_probe.so (an attempt needs half a block);
+`host_loop` is the same workload driven the reference's way (host weight
+update between launches); `cpu_baseline` times the CPU oracle (the same
+chain, threaded over walkers) on a bounded sample of the same workload and
+compares its histograms with the timed batched path's.
 
 `--workload seeded`: GPU seconds to a flat histogram (width == N and
 D_KL < 1e-4, muca_delta_multigpu.cu:441-447) for an unseen (L, T, Delta) =
-(`--L`, 0.55, 1.97), 4096 walkers, trivial update.  "seeded": ln W from the
-surrogate (trained on the GPU on the committed corpus of converged weights,
-L = 16 .. 64, T = 0.5 .. 0.8), NUPDATES from the start = 24 N^2 (`--nup-factor`; the
-reference's starting_M argument); "plain": ln W = 0, NUPDATES = 1 (the
-reference's defaults).  Each run stops at its device-time cap; a run that
-hits it reports a lower bound.
+(`--L` 128, 0.55, 1.97), trivial update, `--jacks` 4 independent jacks of
+`--walkers`/4 walkers each in one batch (default total: the reference's
+walker count for one GPU, maxThreadsPerCU x CUs, muca_delta_multigpu.cu:100-106;
+524 288 on MI355X); a mode's time runs until every jack is flat.  "seeded":
+ln W from the surrogate (trained on the GPU on the committed corpus of
+converged weights; the jacks alternate between two networks), stratified
+start, NUPDATES from `--s0` sweeps growing x`--growth` per iteration up to
+`--smax` sweeps; "nonet": the same from ln W = 0; "plain": the reference's
+defaults (ln W = 0, NUPDATES = 1, x1.01 once width == N, random start).
+Each mode stops at its device-time cap; one that hits it reports a lower
+bound.  ln W accuracy: the per-bin jackknife error over the jacks.
 
 `--workload c5`: large production, L = 256 (65 536 spins), 16 384 walkers,
 T = 0.5, produce keying and FP64 acceptance, ln W from the surrogate, one
@@ -45,7 +53,7 @@ Delta = 1.98 is checked against the CPU oracle's long-double sums.
 
     python bench.py [--steps K] [--warmup W] [--walkers 4096] [--L 72]
     python bench.py --workload batch96 [--walkers 4096]
-    python bench.py --workload seeded [--L 128] [--mode both|seeded|plain]
+    python bench.py --workload seeded [--L 128] [--mode all|both|seeded|plain|nonet]
     python bench.py --workload c5 [--L 256] [--walkers 16384]
     python bench.py --copy-probe BYTES   # plain copy kernel for counter scaling
 """
@@ -63,6 +71,73 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 ALGO_BYTES_PER_ATTEMPT = 2.0   # one int8 spin read + one write per site visit
 METRIC = "flip attempts/ns; GPU s to flat H(M) at L=128, surrogate-seeded vs plain"
+# the sources that define the team update kernel; the committed counter file
+# (profiles/update_kernel_L72_W4096_traffic.json) records their hash, and its
+# figures are reported only while the hash still matches
+TEAM_KERNEL_SOURCES = ["team_run.h", "muca_team.hip", "kernels.h", "philox.h", "acclog.h",
+                       "acclog_table.h"]
+
+
+def team_kernel_source_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for f in TEAM_KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "muca_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def valu_ceiling(rounds=7, grid=2048, iters=20000, reps=5):
+    """The Philox4x32-R block rate of muca_amd/libmuca_probe.so's
+    microbenchmark (every lane of a full-occupancy grid draws the two keyed
+    streams of a walker and nothing else).  An attempt needs half a block, so
+    twice the block rate bounds any update kernel's attempts/ns."""
+    import ctypes
+    path = os.path.join(ROOT, "muca_amd", "libmuca_probe.so")
+    lib = ctypes.CDLL(path)
+    f = lib.muca_probe_philox_rate
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]
+    bpn, ms = ctypes.c_double(), ctypes.c_float()
+    rc = f(0, rounds, grid, iters, reps, ctypes.byref(bpn), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"muca_probe_philox_rate failed with HIP error {rc}")
+    return {"philox_blocks_per_ns": bpn.value, "attempts_per_ns": 2.0 * bpn.value,
+            "launch_ms": ms.value, "lanes": grid * 256, "blocks_per_lane": 2 * iters,
+            "rounds": rounds}
+
+
+def host_cpu():
+    """Threads for the CPU baseline (this process's CPUs, at most the 16 a
+    one-GPU share of the host holds) and the CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(avail, 16), avail, model
+
+
+def batch_hists(M, ins, n):
+    """The timed batched path (same instance, graph-replayed iterations) run
+    again from its start, returning the D histogram of each of its first n
+    iterations."""
+    out = []
+    with M.Batch(ins) as b:
+        b.init(0)
+        for _ in range(n):
+            b.iterate(1, use_graph=True)
+            out.append(b.get_hist(0))
+    return out
 
 
 def copy_probe(nbytes):
@@ -204,17 +279,25 @@ def main():
     ap.add_argument("--blocks", type=int, default=10, help="c5: jackknife blocks")
     ap.add_argument("--observables", choices=["moments", "joint"], default="moments",
                     help="c5: per-D moments (default) or the reference's joint H(D,M)")
-    ap.add_argument("--mode", choices=["both", "seeded", "plain"], default="both")
+    ap.add_argument("--mode", choices=["all", "both", "seeded", "plain", "nonet"],
+                    default="both")
     ap.add_argument("--T", type=float, default=0.55)
     ap.add_argument("--delta", type=float, default=1.97)
-    ap.add_argument("--cap-seeded", type=float, default=600.0)
-    ap.add_argument("--cap-plain", type=float, default=600.0)
-    ap.add_argument("--recursive", action="store_true",
-                    help="seeded workload: weight_recursion instead of the trivial update")
-    ap.add_argument("--save-lnw", default="", help="seeded: write the final ln W of each run "
-                    "to <path>_<run>.npy")
-    ap.add_argument("--nup-factor", type=float, default=24.0,
-                    help="seeded run: NUPDATES from the start = factor * N^2")
+    ap.add_argument("--jacks", type=int, default=4, help="seeded workload: independent jacks")
+    ap.add_argument("--nets", type=int, default=2,
+                    help="seeded workload: networks (different initial parameters) the "
+                         "jacks alternate between")
+    ap.add_argument("--s0", type=float, default=50.0,
+                    help="seeded/nonet: NUPDATES of the first iteration, in sweeps")
+    ap.add_argument("--growth", type=float, default=2.0,
+                    help="seeded/nonet: NUPDATES factor per iteration once width == N")
+    ap.add_argument("--smax", type=float, default=3200.0,
+                    help="seeded/nonet: largest NUPDATES, in sweeps")
+    ap.add_argument("--cap-seeded", type=float, default=900.0)
+    ap.add_argument("--cap-plain", type=float, default=900.0)
+    ap.add_argument("--cap-nonet", type=float, default=600.0)
+    ap.add_argument("--save-lnw", default="", help="seeded: write every jack's final ln W "
+                    "to this .npz")
     ap.add_argument("--L", type=int, default=None, help="72 for c2, 128 for seeded")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1234)
@@ -237,7 +320,7 @@ def main():
         sys.exit(2)
 
     import muca_amd as M
-    nth = min(16, os.cpu_count() or 1)
+    nth, cpus_avail, cpu_name = host_cpu()
     recursion = not args.trivial
     if args.workload == "seeded":
         seeded_workload(M, args)
@@ -277,18 +360,29 @@ def main():
     achieved = ALGO_BYTES_PER_ATTEMPT * att_per_launch / avg_kernel_s / 1e9
 
     # HBM bytes and VALU instructions per launch from the committed rocprofv3
-    # counter passes of this exact workload (none for other shapes)
-    traffic, valu = None, None
+    # counter passes of this exact workload (none for other shapes), used
+    # only while the team kernel's sources are the ones they were taken from
+    traffic, valu, counters = None, None, None
     if args.workload == "c2" and args.L == 72 and W == 4096:
         tfile = os.path.join(ROOT, "profiles", "update_kernel_L72_W4096_traffic.json")
         if os.path.exists(tfile):
             t = json.load(open(tfile))
-            traffic = t.get("bytes_per_launch")
+            fresh = t.get("source_sha") == team_kernel_source_sha()
+            counters = {"file": os.path.relpath(tfile, ROOT), "source_sha": t.get("source_sha"),
+                        "current_source_sha": team_kernel_source_sha(), "fresh": fresh}
             insts = t.get("valu_wave_insts_per_launch")
-            if insts:
+            if fresh:
+                traffic = t.get("bytes_per_launch")
+            if fresh and insts:
                 peak = 256 * 2.4e9   # wave64 VALU instructions/s: 256 CUs x 1/clk x 2.4 GHz
                 valu = {"achieved": insts / avg_kernel_s, "peak": peak,
                         "unit": "wave instructions/s", "frac": insts / avg_kernel_s / peak}
+    kernel_rate = att_per_launch / avg_kernel_s / 1e9
+    ceil = valu_ceiling()
+    roofline_valu = {"bound": "valu: Philox4x32-7 block rate (microbenchmark, this run)",
+                     "achieved": kernel_rate, "peak": ceil["attempts_per_ns"],
+                     "unit": "attempts/ns", "frac": kernel_rate / ceil["attempts_per_ns"],
+                     "probe": ceil}
 
     host = None
     cpu = None
@@ -301,15 +395,19 @@ def main():
         if not args.no_cpu_baseline:
             att, el, iters, chists = cpu_weight_loop(args.L, W, args.seed, recursion,
                                                      args.cpu_budget, nth)
-            same = None
-            if host is not None:
-                n = min(len(hhists), len(chists))
-                same = all(np.array_equal(a, b) for a, b in zip(hhists[:n], chists[:n]))
+            # the timed path itself, re-run from its start for as many iterations
+            ghists = batch_hists(M, ins, iters)
+            same = 0
+            while same < iters and np.array_equal(ghists[same], chists[same]):
+                same += 1
             cpu = {"value": att / el / 1e9, "unit": "attempts/ns", "cores": nth, "kind": "port",
-                   "sample": f"CPU oracle (same chain, {nth} threads over walkers) on the first "
-                             f"{iters} weight iterations of this workload ({att} attempts, "
-                             f"{el:.1f} s); histograms identical to the GPU host-loop run's "
-                             f"for those iterations: {same}"}
+                   "cpu": cpu_name, "cpus_available": cpus_avail,
+                   "sample": f"CPU oracle (the same chain; {nth} threads over walkers; host "
+                             f"weight update) on the first {iters} weight iterations of this "
+                             f"workload ({att} attempts, {el:.1f} s); the D histograms of the "
+                             f"timed batched GPU path (re-run from its start) are identical "
+                             f"to the oracle's for the first {same} of those {iters} "
+                             f"iterations"}
     elif not args.no_cpu_baseline:
         # a bounded sample: first weight iteration of instances 0, 1, ... (L = 32) until
         # the budget is spent
@@ -329,6 +427,7 @@ def main():
                 break
         el = time.perf_counter() - t0
         cpu = {"value": att / el / 1e9, "unit": "attempts/ns", "cores": nth, "kind": "port",
+               "cpu": cpu_name, "cpus_available": cpus_avail,
                "sample": f"CPU oracle (same chain, {nth} threads over walkers): first weight "
                          f"iteration (NUPDATES = 1) of instances 0..{done - 1}, {att} attempts, "
                          f"{el:.1f} s"}
@@ -354,7 +453,9 @@ def main():
                   "autotune": r["tune"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "roofline_valu": roofline_valu,
         "valu_issue": valu,
+        "counter_file": counters,
         "host_loop": host,
         "cpu_baseline": cpu,
     }
@@ -366,86 +467,127 @@ def main():
 
 
 def seeded_workload(M, args):
+    """GPU seconds to a flat histogram for an unseen (L, T, Delta), surrogate
+    seeded against the method without the network and the reference's plain
+    method, each as `--jacks` independent jacks in one batch."""
     from muca_amd import surrogate as S
     L = args.L or 128
-    W = args.walkers or 4096
+    Wtot = args.walkers or M.default_walkers(0)
+    J = args.jacks
+    W = Wtot // J
     beta = 1.0 / args.T
     N = L * L
+    modes = ["seeded", "nonet", "plain"] if args.mode == "all" else (
+        ["seeded", "plain"] if args.mode == "both" else [args.mode])
+    caps = {"seeded": args.cap_seeded, "nonet": args.cap_nonet, "plain": args.cap_plain}
 
     def log(tag):
         def f(gpu, st):
-            s = st[0]
-            print(f"{tag}: gpu {gpu:.1f}s iter {s['iter']} width {s['width']} D_KL "
-                  f"{s['dkl']:.2e} NUPDATES {s['nup']}", file=sys.stderr, flush=True)
+            print(f"{tag}: gpu {gpu:.1f}s " + " ".join(
+                f"[it {x['iter']} w {x['width']} dkl {x['dkl']:.2e} nup {x['nup']} "
+                f"{'flat' if x['converged'] else ''}]" for x in st), file=sys.stderr, flush=True)
         return f
 
-    res = {}
-    train_info = None
-    if args.mode in ("both", "seeded"):
+    train_info, seeds = [], None
+    if "seeded" in modes:
+        # jacks alternate between networks trained from different initial
+        # parameters, so a flat ln W that depended on its seed would show as a
+        # difference between the two groups
         X, t, w = S.corpus(S.load_corpus())
-        net, train_info = S.train(X, t, w)
-        lnw0 = S.seed_log_weight(net, L, beta, args.delta)
-        net.close()
-        nup0 = S.equilibrium_nupdates(L, args.nup_factor)
-        print(f"seeded: trained on {len(X)} samples in {train_info['gpu_s']:.2f} GPU s, "
-              f"NUPDATES from the start {nup0}", file=sys.stderr, flush=True)
-        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta,
-                                  recursion=args.recursive, init_nup=nup0)],
-                            lnw_init=[lnw0], stratified=False,
-                            max_gpu_s=args.cap_seeded, chunk=1, log=log("seeded"))
-        res["seeded"] = out[0]
-    if args.mode in ("both", "plain"):
-        out, _ = S.converge([dict(L=L, walkers=W, seed=777, beta=beta,
-                                  recursion=args.recursive, init_nup=1)], stratified=False,
-                            max_gpu_s=args.cap_plain,
-                            chunk=50, log=log("plain"))
-        res["plain"] = out[0]
+        nets = []
+        for k in range(args.nets):
+            net, info = S.train(X, t, w, seed=k)
+            nets.append(S.seed_log_weight(net, L, beta, args.delta))
+            net.close()
+            train_info.append({kk: v for kk, v in info.items() if kk != "loss_curve"})
+            print(f"seeded: network {k} trained on {info['samples']} samples in "
+                  f"{info['gpu_s']:.2f} GPU s", file=sys.stderr, flush=True)
+        seeds = [nets[j % args.nets] for j in range(J)]
+    res = {}
+    for mode in modes:
+        ins = S.protocol_instances(L, beta, J, W, mode, s0=args.s0, growth=args.growth,
+                                   smax=args.smax)
+        out, gpu = S.converge(ins, lnw_init=seeds if mode == "seeded" else None,
+                              stratified=mode != "plain", max_gpu_s=caps[mode],
+                              chunk=1 if mode != "plain" else 50, log=log(mode))
+        res[mode] = dict(gpu_s=gpu, out=out)
+        print(f"{mode}: {sum(o['status']['converged'] for o in out)}/{J} jacks flat in "
+              f"{gpu:.1f} GPU s", file=sys.stderr, flush=True)
 
-    def summary(o):
-        st = o["status"]
-        return {"flat": bool(st["converged"]), "gpu_s": o["gpu_s"], "iterations": st["iter"],
-                "width_reached_N_at": None if st["reach_iter"] == 0xFFFFFFFF else st["reach_iter"],
-                "last_D_KL": st["dkl"], "attempts": st["attempts"]}
+    def summary(r):
+        st = [o["status"] for o in r["out"]]
+        return {"flat": all(x["converged"] for x in st), "gpu_s": r["gpu_s"],
+                "jacks_flat": int(sum(x["converged"] for x in st)),
+                "iterations": [x["iter"] for x in st],
+                "width_reached_N_at": [None if x["reach_iter"] == 0xFFFFFFFF else x["reach_iter"]
+                                       for x in st],
+                "last_D_KL": [x["dkl"] for x in st],
+                "attempts": int(sum(x["attempts"] for x in st))}
 
     sm = {k: summary(v) for k, v in res.items()}
     if args.save_lnw:
-        for k, v in res.items():
-            np.save(f"{args.save_lnw}_{k}.npy", v["lnw"])
-    ratio = None
-    if "seeded" in sm and "plain" in sm and sm["seeded"]["flat"]:
-        r = sm["plain"]["gpu_s"] / sm["seeded"]["gpu_s"]
-        ratio = {"value": r, "lower_bound": not sm["plain"]["flat"]}
-    agree = None
-    if "seeded" in res and "plain" in res and sm["seeded"]["flat"] and sm["plain"]["flat"]:
-        d = res["seeded"]["lnw"] - res["plain"]["lnw"]
-        agree = {"max_abs_lnw_diff": float(np.max(np.abs(d))),
-                 "rms_lnw_diff": float(np.sqrt(np.mean(d * d)))}
-    main_run = sm.get("seeded") or sm.get("plain")
+        np.savez_compressed(args.save_lnw, **{f"{k}_jack{j}": o["lnw"] for k, v in res.items()
+                                              for j, o in enumerate(v["out"])})
+    speed = {}
+    if "seeded" in sm and sm["seeded"]["flat"]:
+        for other in ("plain", "nonet"):
+            if other in sm:
+                speed[f"{other}_over_seeded"] = {
+                    "value": sm[other]["gpu_s"] / sm["seeded"]["gpu_s"],
+                    "lower_bound": not sm[other]["flat"]}
+    acc = {}
+    for k, v in res.items():
+        if not sm[k]["flat"] or J < 2:
+            continue
+        lnws = [o["lnw"] for o in v["out"]]
+        mean, err = S.jackknife_lnw(lnws)
+        rec = {"jackknife_err_median": float(np.median(err[1:])),
+               "jackknife_err_max": float(np.max(err[1:]))}
+        if k == "seeded" and args.nets == 2 and J >= 4:
+            ga, gb = np.mean(lnws[0::2], axis=0), np.mean(lnws[1::2], axis=0)
+            ea = np.std(lnws[0::2], axis=0, ddof=1) / np.sqrt(len(lnws[0::2]))
+            eb = np.std(lnws[1::2], axis=0, ddof=1) / np.sqrt(len(lnws[1::2]))
+            z = (ga - gb)[1:] / np.sqrt(ea[1:] ** 2 + eb[1:] ** 2)
+            rec["network_groups_rms_z"] = float(np.sqrt(np.mean(z * z)))
+        acc[k] = rec
+        acc[k + "_mean"] = mean
+    if "seeded" in acc and "plain" in acc:
+        d = (acc["seeded_mean"] - acc["plain_mean"])[1:]
+        e = np.sqrt(S.jackknife_lnw([o["lnw"] for o in res["seeded"]["out"]])[1] ** 2 +
+                    S.jackknife_lnw([o["lnw"] for o in res["plain"]["out"]])[1] ** 2)[1:]
+        acc["seeded_vs_plain"] = {"rms_diff": float(np.sqrt(np.mean(d * d))),
+                                  "rms_z": float(np.sqrt(np.mean((d / e) ** 2)))}
+    for k in list(acc):
+        if k.endswith("_mean"):
+            del acc[k]
+    main_run = sm.get("seeded") or sm[modes[0]]
     out = {
         "metric": METRIC,
         "value": main_run["gpu_s"],
         "unit": "GPU s to flat H",
         "n_gpus": 1,
-        "steps": main_run["iterations"],
+        "steps": int(max(main_run["iterations"])),
         "warmup": 0,
-        "ms_per_step": main_run["gpu_s"] * 1e3 / max(1, main_run["iterations"]),
+        "ms_per_step": main_run["gpu_s"] * 1e3 / max(1, max(main_run["iterations"])),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int8 spins, f32 acceptance, f64 weight update; surrogate bf16 MFMA / f32",
-        "data": "synthetic (random initial spins from the reference's Philox keying)",
+        "data": "synthetic (Philox-keyed initial spins)",
         "config": {"workload": f"weight iteration to flat H, L={L}, T={args.T}, "
-                               f"Delta={args.delta}, {W} walkers, "
-                               f"{'recursive' if args.recursive else 'trivial'} update",
-                   "seeded_nupdates": f"{args.nup_factor} N^2",
-                   "L": L, "T": args.T, "delta": args.delta, "walkers": W,
-                   "criterion": "width == N and D_KL < 1e-4",
-                   "caps_gpu_s": {"seeded": args.cap_seeded, "plain": args.cap_plain}},
+                               f"Delta={args.delta}, {J} jacks x {W} walkers, trivial update",
+                   "L": L, "T": args.T, "delta": args.delta, "jacks": J,
+                   "walkers_per_jack": W,
+                   "criterion": "every jack: width == N and D_KL < 1e-4",
+                   "seeded_schedule": f"NUPDATES {args.s0} N, x{args.growth} per iteration, "
+                                      f"up to {args.smax} N; stratified start",
+                   "plain": "ln W = 0, NUPDATES = 1, x1.01 once width == N, random start",
+                   "nonet": "ln W = 0 with the seeded schedule and start",
+                   "caps_gpu_s": {m: caps[m] for m in modes}},
         "runs": sm,
-        "speedup_plain_over_seeded": ratio,
-        "lnw_agreement": agree,
-        "surrogate_training": None if train_info is None else
-        {k: v for k, v in train_info.items() if k != "loss_curve"},
+        "speedup": speed,
+        "lnw_accuracy": acc,
+        "surrogate_training": train_info,
     }
     print(json.dumps(out))
 

@@ -204,6 +204,10 @@ typedef struct muca_instance {
     uint64_t init_nup;   /* NUPDATES of the first iteration (0: 1); larger than
                             MUCA_MAX_NUPDATES: MUCA_E_RANGE from muca_batch_create */
     double beta;         /* 2.0 reproduces the reference's -dE*2 */
+    double nup_growth;   /* once width == N each iteration sets NUPDATES =
+                            (NUPDATES + 1) * nup_growth; 0 means 1.01, the
+                            reference's schedule (:379-382); else >= 1 */
+    uint64_t nup_max;    /* the schedule stops here (0: MUCA_MAX_NUPDATES) */
 } muca_instance;
 
 typedef struct muca_instance_status {
@@ -213,8 +217,8 @@ typedef struct muca_instance_status {
     uint32_t reach_iter; /* iteration at which width reached N (0xFFFFFFFF: not yet) */
     uint32_t done_iter;  /* iteration that converged (0xFFFFFFFF: not yet) */
     int32_t start, end;  /* hist_range of the last iteration */
-    uint32_t nup_capped; /* 1 once the schedule would have passed MUCA_MAX_NUPDATES
-                            (NUPDATES then stays at that maximum) */
+    uint32_t nup_capped; /* 1 once the schedule reached the instance's nup_max
+                            (NUPDATES then stays there) */
     uint64_t nup;        /* NUPDATES of the next iteration */
     double dkl;          /* Kullback-Leibler divergence of the last iteration */
     uint64_t attempts;   /* flip attempts done (therm + nupdates per walker) */
@@ -222,7 +226,10 @@ typedef struct muca_instance_status {
 
 const char *muca_batch_last_error(void);
 /* tiles: MUCA_KERNEL_AUTO, MUCA_KERNEL_TEAM (every L % 8 == 0, L <= 128) or
- * MUCA_KERNEL_LANE; precision: MUCA_REF32 | MUCA_FP64 */
+ * MUCA_KERNEL_LANE; precision: MUCA_REF32 | MUCA_FP64.  Lane tiles hold
+ * 256 walkers, or 512 / 1024 when the largest instance's ln W and histogram
+ * leave LDS room for only a few workgroups per compute unit (the autotuner
+ * may choose otherwise). */
 int muca_batch_create(int32_t device, const muca_instance *instances, uint32_t n,
                       int32_t precision, int32_t tiles, muca_batch **batch);
 int muca_batch_destroy(muca_batch *batch);
@@ -260,9 +267,10 @@ int muca_batch_iterate_profiled(muca_batch *batch, uint32_t iterations, float *u
  * 256 threads per block, muca_delta_multigpu.cu:27): times one iteration of
  * every eligible tile kind and workgroup size from freshly initialised walkers
  * (muca_batch_init(mm)), keeps the fastest and leaves the batch initialised
- * with mm, except that every instance's ln W is what it was before the call
- * (so weights set with muca_batch_set_log_weight survive; call order create,
- * init, set_log_weight, autotune, iterate).  kind/block/ms (nullable) report
+ * with mm, except that in a batch initialised before the call every
+ * instance's ln W is what it was before the call (so weights set with
+ * muca_batch_set_log_weight survive; call order create, init,
+ * set_log_weight, autotune, iterate).  kind/block/ms (nullable) report
  * the choice. */
 int muca_batch_autotune(muca_batch *batch, uint32_t mm, int32_t *kind, uint32_t *block,
                         float *ms);

@@ -60,7 +60,8 @@ MAX_NUPDATES = 1 << 48           # include/muca.h MUCA_MAX_NUPDATES
 class Instance(ctypes.Structure):
     _fields_ = [("L", ctypes.c_uint32), ("walkers", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("recursion", ctypes.c_int32),
-                ("init_nup", ctypes.c_uint64), ("beta", ctypes.c_double)]
+                ("init_nup", ctypes.c_uint64), ("beta", ctypes.c_double),
+                ("nup_growth", ctypes.c_double), ("nup_max", ctypes.c_uint64)]
 
 
 class InstanceStatus(ctypes.Structure):
@@ -79,14 +80,15 @@ def pack_instances(instances):
     arr = (Instance * len(ins))()
     for i, q in enumerate(ins):
         vals = {"L": q["L"], "walkers": q["walkers"], "seed": q["seed"],
-                "init_nup": q.get("init_nup", 1)}
+                "init_nup": q.get("init_nup", 1), "nup_max": q.get("nup_max", 0)}
         for k, v in vals.items():
-            hi = MAX_NUPDATES if k == "init_nup" else 0xFFFFFFFF
+            hi = MAX_NUPDATES if k in ("init_nup", "nup_max") else 0xFFFFFFFF
             if int(v) != v or not 0 <= v <= hi:
                 raise MucaError(f"instance {i}: {k} = {v} is outside [0, {hi}]")
         arr[i] = Instance(int(q["L"]), int(q["walkers"]), int(q["seed"]),
                           1 if q.get("recursion") else 0, int(vals["init_nup"]),
-                          float(q.get("beta", 2.0)))
+                          float(q.get("beta", 2.0)), float(q.get("nup_growth", 0.0)),
+                          int(vals["nup_max"]))
     return arr
 
 
@@ -323,7 +325,9 @@ class Batch:
     muca_delta_multigpu.cu:314-476); the loop body runs on the device.
 
     instances: iterable of dicts with keys L, walkers, seed, and optionally
-    beta (2.0), init_nup (1, at most MAX_NUPDATES), recursion (False)."""
+    beta (2.0), init_nup (1, at most MAX_NUPDATES), recursion (False),
+    nup_growth (0: the reference's 1.01) and nup_max (0: MAX_NUPDATES) of the
+    NUPDATES schedule once the histogram spans all bins."""
 
     def __init__(self, instances, device=0, *, precision=REF32, tiles=KERNEL_AUTO):
         ins = list(instances)

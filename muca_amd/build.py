@@ -6,6 +6,7 @@ Outputs (git-ignored, shipped with the tree):
     muca_amd/libmuca.so               the C-ABI library (include/muca.h)
     muca_amd/bin/multigpu_weight      drop-in for muca_delta_multigpu.cu's binary
     muca_amd/bin/produce_gpu          drop-in for produce2_multigpu.cu's binary
+    muca_amd/libmuca_probe.so         VALU-ceiling microbenchmarks for bench.py
 """
 import os
 import shutil
@@ -83,10 +84,22 @@ def build_cli(verbose=False):
     return out
 
 
+def build_probe(verbose=False):
+    """libmuca_probe.so: microbenchmarks bench.py prices the update kernels
+    against (not part of the libmuca ABI)."""
+    out = os.path.join(PKG, "libmuca_probe.so")
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "probe_valu.hip"), "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose=False):
     lib = build_lib(verbose)
     clis = build_cli(verbose)
-    return [lib] + clis
+    return [lib] + clis + [build_probe(verbose)]
 
 
 if __name__ == "__main__":

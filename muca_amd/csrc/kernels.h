@@ -82,13 +82,15 @@ struct BatchInst {
     uint64_t w_off;        // bins into ln W / histogram / scratch arrays
     double beta;
     int32_t recursion;     // 0 trivial update, 1 weight_recursion
+    double nup_growth;     // schedule factor once width == N (1.01: the reference)
+    uint64_t nup_max;      // schedule cap
     uint32_t lane_lds_w;   // lane tiles: LDS bytes of ln W
     uint32_t lane_hist;    // lane tiles: histogram in LDS
     TeamLayout lay;        // team tiles
     // advanced on the device
     uint32_t iter, width, reach, converged, reach_iter, done_iter;
     int32_t start, end;
-    uint32_t nup_capped;   // the schedule reached MUCA_MAX_NUPDATES
+    uint32_t nup_capped;   // the schedule reached nup_max
     uint64_t nup;          // NUPDATES of the current iteration
     double dkl;
     uint64_t attempts;     // flip attempts done so far

@@ -208,12 +208,13 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
             }
             if (threadIdx.x == 0) {
                 if (I->width == N) {                                 // :379-382, next iteration
-                    // 64-bit NUPDATES: equal to the reference's unsigned
-                    // arithmetic below 2^32, capped (never 0) above
+                    // 64-bit NUPDATES: with growth 1.01 equal to the
+                    // reference's unsigned arithmetic below 2^32; capped
+                    // (never 0) at the instance's maximum
                     const uint64_t nu = I->nup + 1;
-                    const double grown = (double)nu * 1.01;
-                    if (grown >= (double)MUCA_MAX_NUPDATES) {
-                        I->nup = MUCA_MAX_NUPDATES;
+                    const double grown = (double)nu * I->nup_growth;
+                    if (grown >= (double)I->nup_max) {
+                        I->nup = I->nup_max;
                         I->nup_capped = 1;
                     } else {
                         I->nup = (uint64_t)grown;
@@ -279,6 +280,7 @@ struct muca_batch {
     bool team_ok = false, lane_ok = false;
     size_t lds_device = 0;
     int stratified = 0;   // start of the last init
+    bool inited = false;  // muca_batch_init* ran (ln W holds values, not garbage)
 };
 
 namespace {
@@ -325,6 +327,19 @@ hipError_t configure(muca_batch *b, int kind, uint32_t block)
     b->exec = nullptr;
     b->graph = nullptr;
     b->kind = kind;
+    size_t need = 0;   // LDS of the largest instance's tile
+    for (const auto &I : b->inst)
+        need = std::max(need, kind == MUCA_KERNEL_TEAM
+                                  ? I.lay.bytes
+                                  : (size_t)I.lane_lds_w + (I.lane_hist ? (size_t)(I.N + 1) * 4 : 0));
+    const uint32_t per_cu_lds = (uint32_t)std::max<size_t>(1, b->lds_device / (need + 256));
+    if (kind == MUCA_KERNEL_LANE && block == 0) {
+        // one lane per walker hides latency with many wavefronts: when LDS
+        // admits few workgroups per compute unit, make them larger (L = 128:
+        // ln W and the histogram take 128 KB, one 1024-lane workgroup per CU)
+        block = 256;
+        while (block < 1024 && per_cu_lds * block < 2048) block *= 2;
+    }
     b->block = kind == MUCA_KERNEL_TEAM ? 256 : block;
     const uint32_t per = kind == MUCA_KERNEL_TEAM ? 16 : b->block;
     const uint32_t n = (uint32_t)b->inst.size();
@@ -335,13 +350,7 @@ hipError_t configure(muca_batch *b, int kind, uint32_t block)
     b->tiles.clear();
     for (uint32_t i : order)
         for (uint32_t w = 0; w < b->inst[i].W; w += per) b->tiles.push_back({i, w});
-    size_t need = 0;
-    for (const auto &I : b->inst)
-        need = std::max(need, kind == MUCA_KERNEL_TEAM
-                                  ? I.lay.bytes
-                                  : (size_t)I.lane_lds_w + (I.lane_hist ? (size_t)(I.N + 1) * 4 : 0));
     b->lds_bytes = need;
-    const uint32_t per_cu_lds = (uint32_t)std::max<size_t>(1, b->lds_device / (need + 256));
     const uint32_t per_cu_thr = 2048 / b->block;
     b->grid = b->cus * std::min(per_cu_lds, per_cu_thr);
     b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->tiles.size());
@@ -375,9 +384,12 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
         if (q.L < 4 || q.L % 2 != 0 || q.L > 46340 || q.walkers == 0 || !(q.beta > 0))
             return bfail(MUCA_E_ARG, "instance " + std::to_string(i) +
                                          ": need even L >= 4, walkers > 0, beta > 0");
-        if (q.init_nup > MUCA_MAX_NUPDATES)
+        if (q.init_nup > MUCA_MAX_NUPDATES || q.nup_max > MUCA_MAX_NUPDATES)
             return bfail(MUCA_E_RANGE, "instance " + std::to_string(i) +
-                                           ": init_nup exceeds MUCA_MAX_NUPDATES (2^48)");
+                                           ": init_nup or nup_max exceeds MUCA_MAX_NUPDATES (2^48)");
+        if (!(q.nup_growth == 0 || (q.nup_growth >= 1 && q.nup_growth <= 1e6)))
+            return bfail(MUCA_E_ARG, "instance " + std::to_string(i) +
+                                         ": nup_growth must be 0 (1.01) or in [1, 1e6]");
     }
     int ndev = 0;
     BHIP(hipGetDeviceCount(&ndev));
@@ -402,6 +414,8 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
         I.seed = q.seed;
         I.beta = q.beta;
         I.recursion = q.recursion ? 1 : 0;
+        I.nup_growth = q.nup_growth == 0 ? 1.01 : q.nup_growth;
+        I.nup_max = q.nup_max == 0 ? MUCA_MAX_NUPDATES : q.nup_max;
         I.lat_off = lat;
         I.st_off = st;
         I.w_off = wo;
@@ -466,7 +480,7 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     if (e == hipSuccess) e = balloc(&A.bf, wo);
     if (e == hipSuccess) e = balloc(&A.p_acc, wo);
     A.tiles = b->d_tiles;
-    if (e == hipSuccess) e = configure(b, kind, 256);
+    if (e == hipSuccess) e = configure(b, kind, 0);
     if (e != hipSuccess) {
         batch_free(b);
         return bfail(MUCA_E_NOMEM, std::string("batch setup: ") + hipGetErrorString(e));
@@ -560,6 +574,7 @@ int batch_init(muca_batch *b, uint32_t mm, int stratified)
     BHIP(hipMemset(A.hist, 0, b->bins * 8));
     BHIP(hipMemset(A.hist_last, 0, b->bins * 8));
     BHIP(hipMemset(A.p_acc, 0, b->bins * 8));
+    b->inited = true;
     return MUCA_OK;
 }
 }  // namespace
@@ -709,13 +724,16 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
     if (b->team_ok) cands.push_back({MUCA_KERNEL_TEAM, 256});
     if (b->lane_ok)
         for (uint32_t blk : {256u, 512u, 1024u}) cands.push_back({MUCA_KERNEL_LANE, blk});
-    // ln W set before the call (a surrogate seed, say) survives the tuning:
-    // each candidate re-initialises the batch, which zeroes ln W
+    // ln W of an initialised batch (a surrogate seed, say) survives the
+    // tuning: each candidate re-initialises the batch, which zeroes ln W
     BHIP(hipStreamSynchronize(b->st));
-    std::vector<double> keep64(b->bins);
-    std::vector<float> keep32(b->bins);
-    BHIP(hipMemcpy(keep64.data(), b->a.lnw64, b->bins * 8, hipMemcpyDeviceToHost));
-    BHIP(hipMemcpy(keep32.data(), b->a.lnw32, b->bins * 4, hipMemcpyDeviceToHost));
+    const bool keep = b->inited;
+    std::vector<double> keep64(keep ? b->bins : 0);
+    std::vector<float> keep32(keep ? b->bins : 0);
+    if (keep) {
+        BHIP(hipMemcpy(keep64.data(), b->a.lnw64, b->bins * 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpy(keep32.data(), b->a.lnw32, b->bins * 4, hipMemcpyDeviceToHost));
+    }
     float best = 0;
     Cand pick{b->kind, b->block};
     for (const Cand &c : cands) {
@@ -733,8 +751,10 @@ int muca_batch_autotune(muca_batch *b, uint32_t mm, int32_t *kind, uint32_t *blo
     BHIP(configure(b, pick.kind, pick.block));
     const int rc = batch_init(b, mm, b->stratified);
     if (rc) return rc;
-    BHIP(hipMemcpy(b->a.lnw64, keep64.data(), b->bins * 8, hipMemcpyHostToDevice));
-    BHIP(hipMemcpy(b->a.lnw32, keep32.data(), b->bins * 4, hipMemcpyHostToDevice));
+    if (keep) {
+        BHIP(hipMemcpy(b->a.lnw64, keep64.data(), b->bins * 8, hipMemcpyHostToDevice));
+        BHIP(hipMemcpy(b->a.lnw32, keep32.data(), b->bins * 4, hipMemcpyHostToDevice));
+    }
     if (kind) *kind = pick.kind;
     if (block) *block = b->block;
     if (ms) *ms = best;

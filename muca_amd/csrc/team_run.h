@@ -69,11 +69,12 @@ __device__ __forceinline__ int32_t team_sum(int32_t v)
     return v;
 }
 
-// value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L)
+// value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L;
+// every lane of the row reads a valid source, so no old value is needed)
 template <int L>
 __device__ __forceinline__ int32_t row_bcast(int32_t v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, 0x150 + L, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xf, 0xf, false);
 }
 
 }  // namespace team_detail
@@ -239,124 +240,131 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     }
                 }
                 wave_sync();
-                // ---- B: the D chain
+                // ---- B: the D chain.  The chain runs on `off` = D - dlo (the
+                // window position); per step one DPP broadcast of the attempt's
+                // mask word, the accept bit at the current position, its dDel,
+                // and the move.  A position outside the window reads a zero bit
+                // (bits kWin..15 and 24..31 of the word are zero), so the chain
+                // then stays put; the D recovered after the round shows such an
+                // excursion, and the round is replayed exactly.
                 uint32_t accb[NR];
                 int32_t dv[NR], mv[NR];
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    accb[r] = 0;
-                    dv[r] = 0;
-                    mv[r] = 0;
-                }
-                const int32_t d_start = d;
-                // One chain step.  CHK = false: no memory access and no branch,
-                // a D outside the window only sets `bad` (the round is then
-                // replayed with CHK = true, which decides such a step exactly
-                // from the attempt's record in LDS).
-                auto step = [&](auto lc, auto chk, int r, uint32_t &ab, bool &bad) {
+                const uint32_t sh = (uint32_t)(tid & 48);    // this team's 16 lanes of a ballot
+                const uint32_t upto = (2u << tl) - 1u;       // steps 0..tl of a round
+                int32_t off = d - dlo;
+                // D after this lane's step from the round's start, the accept bits
+                // and the dDel of every step (two ballots)
+                auto recover = [&](int r, int32_t off0, uint32_t ab) -> int32_t {
+                    const int32_t my = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
+                    const uint32_t up =
+                        (uint32_t)(__builtin_amdgcn_ballot_w64(my > 0) >> sh) & 0xffffu & ab;
+                    const uint32_t dn =
+                        (uint32_t)(__builtin_amdgcn_ballot_w64(my < 0) >> sh) & 0xffffu & ab;
+                    return off0 + __builtin_popcount(up & upto) - __builtin_popcount(dn & upto);
+                };
+                // One chain step.  CHK = false: no memory access and no branch.
+                // CHK = true: a step that starts outside the window is decided
+                // exactly from the attempt's record in LDS.
+                auto step = [&](auto lc, auto chk, int r, uint32_t &ab) {
                     constexpr int l = decltype(lc)::value;
                     constexpr bool CHK = decltype(chk)::value;
                     const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
                     const int32_t ddel = __builtin_amdgcn_sbfe((int32_t)mj, 16, 2);
-                    const uint32_t off = (uint32_t)(d - dlo);
-                    uint32_t acc = (mj >> (off & 15u)) & 1u;
-                    const bool oob = off >= (uint32_t)kWin;
+                    // v_bfe_u32 reads the offset modulo 32
+                    uint32_t acc = __builtin_amdgcn_ubfe(mj, (uint32_t)off, 1);
                     if (CHK) {
-                        if (__builtin_amdgcn_ballot_w64(oob)) {   // D left the window
-                            const TeamRec<T> rc = rec[16 * r + l];
-                            T diff = rc.fde + wt[d + rc.ddel];
-                            diff = diff - wt[d];
-                            const uint32_t slow = (T)rc.logu < diff ? 1u : 0u;
-                            acc = oob ? slow : acc;
+                        const bool oob = (uint32_t)off >= (uint32_t)kWin;
+                        if (__builtin_amdgcn_ballot_w64(oob)) {
+                            if (oob) {
+                                const TeamRec<T> rc = rec[16 * r + l];
+                                const int32_t dd = dlo + off;
+                                T diff = rc.fde + wt[dd + rc.ddel];
+                                diff = diff - wt[dd];
+                                acc = (T)rc.logu < diff ? 1u : 0u;
+                            }
                         }
-                    } else {
-                        bad = bad || oob;
                     }
-                    d += (int32_t)acc * ddel;
+                    off += (int32_t)acc * ddel;
                     ab |= acc << l;
-                    if (JOINT) {
-                        const int32_t ij = row_bcast<l>(info[r]);
-                        dv[r] = tl == l ? d : dv[r];
-                        m += (int32_t)acc * (((ij >> 7) & 7) - 2);
-                        mv[r] = tl == l ? m : mv[r];
-                    }
                 };
                 using Fast = std::false_type;
                 using Checked = std::true_type;
-#define MUCA_STEPS(K, AB, BAD)                                                               \
+#define MUCA_STEPS(K, AB)                                                                    \
     if (left >= 16) {                                                                        \
-        step(std::integral_constant<int, 0>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 1>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 2>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 3>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 4>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 5>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 6>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 7>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 8>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 9>(), K(), r, AB, BAD);                             \
-        step(std::integral_constant<int, 10>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 11>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 12>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 13>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 14>(), K(), r, AB, BAD);                            \
-        step(std::integral_constant<int, 15>(), K(), r, AB, BAD);                            \
+        step(std::integral_constant<int, 0>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 1>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 2>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 3>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 4>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 5>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 6>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 7>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 8>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 9>(), K(), r, AB);                                  \
+        step(std::integral_constant<int, 10>(), K(), r, AB);                                 \
+        step(std::integral_constant<int, 11>(), K(), r, AB);                                 \
+        step(std::integral_constant<int, 12>(), K(), r, AB);                                 \
+        step(std::integral_constant<int, 13>(), K(), r, AB);                                 \
+        step(std::integral_constant<int, 14>(), K(), r, AB);                                 \
+        step(std::integral_constant<int, 15>(), K(), r, AB);                                 \
     } else {                                                                                 \
-        if (0 < left) step(std::integral_constant<int, 0>(), K(), r, AB, BAD);               \
-        if (1 < left) step(std::integral_constant<int, 1>(), K(), r, AB, BAD);               \
-        if (2 < left) step(std::integral_constant<int, 2>(), K(), r, AB, BAD);               \
-        if (3 < left) step(std::integral_constant<int, 3>(), K(), r, AB, BAD);               \
-        if (4 < left) step(std::integral_constant<int, 4>(), K(), r, AB, BAD);               \
-        if (5 < left) step(std::integral_constant<int, 5>(), K(), r, AB, BAD);               \
-        if (6 < left) step(std::integral_constant<int, 6>(), K(), r, AB, BAD);               \
-        if (7 < left) step(std::integral_constant<int, 7>(), K(), r, AB, BAD);               \
-        if (8 < left) step(std::integral_constant<int, 8>(), K(), r, AB, BAD);               \
-        if (9 < left) step(std::integral_constant<int, 9>(), K(), r, AB, BAD);               \
-        if (10 < left) step(std::integral_constant<int, 10>(), K(), r, AB, BAD);             \
-        if (11 < left) step(std::integral_constant<int, 11>(), K(), r, AB, BAD);             \
-        if (12 < left) step(std::integral_constant<int, 12>(), K(), r, AB, BAD);             \
-        if (13 < left) step(std::integral_constant<int, 13>(), K(), r, AB, BAD);             \
-        if (14 < left) step(std::integral_constant<int, 14>(), K(), r, AB, BAD);             \
+        if (0 < left) step(std::integral_constant<int, 0>(), K(), r, AB);                    \
+        if (1 < left) step(std::integral_constant<int, 1>(), K(), r, AB);                    \
+        if (2 < left) step(std::integral_constant<int, 2>(), K(), r, AB);                    \
+        if (3 < left) step(std::integral_constant<int, 3>(), K(), r, AB);                    \
+        if (4 < left) step(std::integral_constant<int, 4>(), K(), r, AB);                    \
+        if (5 < left) step(std::integral_constant<int, 5>(), K(), r, AB);                    \
+        if (6 < left) step(std::integral_constant<int, 6>(), K(), r, AB);                    \
+        if (7 < left) step(std::integral_constant<int, 7>(), K(), r, AB);                    \
+        if (8 < left) step(std::integral_constant<int, 8>(), K(), r, AB);                    \
+        if (9 < left) step(std::integral_constant<int, 9>(), K(), r, AB);                    \
+        if (10 < left) step(std::integral_constant<int, 10>(), K(), r, AB);                  \
+        if (11 < left) step(std::integral_constant<int, 11>(), K(), r, AB);                  \
+        if (12 < left) step(std::integral_constant<int, 12>(), K(), r, AB);                  \
+        if (13 < left) step(std::integral_constant<int, 13>(), K(), r, AB);                  \
+        if (14 < left) step(std::integral_constant<int, 14>(), K(), r, AB);                  \
     }
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
+                    accb[r] = 0;
+                    dv[r] = d;
+                    mv[r] = m;
                     const uint32_t j0 = 16 * r;
-                    if (j0 >= cnt) break;
+                    if (j0 >= cnt) continue;
                     const uint32_t left = cnt - j0;
-                    const int32_t d_r0 = d, m_r0 = m;
+                    const int32_t off0 = off;
                     uint32_t ab = 0;
-                    bool bad = false;
-                    MUCA_STEPS(Fast, ab, bad)
-                    if (__builtin_amdgcn_ballot_w64(bad)) {   // rare: replay exactly
-                        if (bad) {
-                            d = d_r0;
-                            m = m_r0;
+                    MUCA_STEPS(Fast, ab)
+                    int32_t o = recover(r, off0, ab);
+                    // a step started outside the window iff the round did or some
+                    // step of it ended there (a lane past the half-row's end
+                    // repeats the last step's D)
+                    const bool out = (uint32_t)off0 >= (uint32_t)kWin || (uint32_t)o >= (uint32_t)kWin;
+                    if (__builtin_amdgcn_ballot_w64(out)) {   // rare: replay exactly
+                        const uint64_t mine = __builtin_amdgcn_ballot_w64(out) >> sh;
+                        if (mine & 0xffffu) {
+                            off = off0;
                             ab = 0;
-                            bool unused = false;
-                            MUCA_STEPS(Checked, ab, unused)
+                            MUCA_STEPS(Checked, ab)
+                            o = recover(r, off0, ab);
                         }
                     }
                     accb[r] = ab;
-                }
-#undef MUCA_STEPS
-                if (!JOINT) {
-                    // D after each lane's own attempt: the chain's start plus the
-                    // dDel of the accepted attempts up to it, counted from the
-                    // team's 16-bit slices of two ballots
-                    const uint32_t sh = (uint32_t)(tid & 48);
-                    const uint32_t upto = (2u << tl) - 1u;
-                    int32_t drun = d_start;
-#pragma unroll
-                    for (int r = 0; r < NR; r++) {
-                        const int32_t my = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
-                        const uint32_t up = (uint32_t)(__builtin_amdgcn_ballot_w64(my > 0) >> sh) &
-                                            0xffffu & accb[r];
-                        const uint32_t dn = (uint32_t)(__builtin_amdgcn_ballot_w64(my < 0) >> sh) &
-                                            0xffffu & accb[r];
-                        dv[r] = drun + __builtin_popcount(up & upto) - __builtin_popcount(dn & upto);
-                        drun += __builtin_popcount(up) - __builtin_popcount(dn);
+                    dv[r] = dlo + o;
+                    d = dlo + off;
+                    if (JOINT) {
+                        // M after this lane's step: an inclusive scan over the
+                        // team of the accepted spin changes
+                        int32_t x = (int32_t)((ab >> tl) & 1u) * (((info[r] >> 7) & 7) - 2);
+                        x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+                        x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+                        x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);
+                        x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);
+                        mv[r] = m + x;
+                        m += row_bcast<15>(x);
                     }
                 }
+#undef MUCA_STEPS
                 // ---- C: accepted spins, dE / dM sums, histogram
                 int32_t de = 0, dm = 0;
 #pragma unroll

@@ -35,12 +35,13 @@ CORPUS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
 
 
 def features(m, L, beta, delta):
+    """delta: one value or one per sample."""
     m = np.asarray(m, np.float64)
     x = np.empty((m.size, 4), np.float32)
     x[:, 0] = m
     x[:, 1] = 16.0 / L
     x[:, 2] = 2.0 * (beta - 1.5)
-    x[:, 3] = 10.0 * (delta - 1.95)
+    x[:, 3] = 10.0 * (np.asarray(delta, np.float64) - 1.95)
     return x
 
 
@@ -135,23 +136,25 @@ def generate_corpus(path, Ls=(16, 24, 32, 40, 48), Ts=(0.5, 0.6, 0.7, 0.8), walk
     return keep, gpu
 
 
-def extend_corpus(path, Ls, Ts, *, walkers=4096, factor=16.0, max_gpu_s=600.0, log=None):
+def extend_corpus(path, Ls, Ts, *, walkers=131072, s0=50.0, growth=2.0, smax=3200.0,
+                  max_gpu_s=600.0, log=None):
     """Closes the loop: a network trained on the corpus at `path` seeds new
-    (L, T) instances, which the batched engine converges with the seeded
-    protocol (trivial update, NUPDATES = factor N^2 from the start, the
-    reference's random start); the flat ones join the corpus.  Returns the
-    keys added and the device seconds spent."""
+    (L, T) instances, which the batched engine converges in one batch with
+    the seeded protocol of protocol_instances (trivial update, stratified
+    start, NUPDATES from s0 sweeps growing x growth per iteration up to smax
+    sweeps); the flat ones join the corpus.  Returns the keys added, the
+    device seconds spent and the per-instance results."""
     X, t, w = corpus(load_corpus(path))
     net, _ = train(X, t, w)
     ins, seeds, meta = [], [], []
     for L in Ls:
         for T in Ts:
-            ins.append(dict(L=L, walkers=walkers, seed=200 + len(ins), beta=1.0 / T,
-                            recursion=False, init_nup=equilibrium_nupdates(L, factor)))
-            seeds.append(seed_log_weight(net, L, 1.0 / T, DELTAS[0]))
+            ins += protocol_instances(L, 1.0 / T, 1, walkers, "seeded", s0=s0, growth=growth,
+                                      smax=smax, seed0=200 + len(ins))
+            seeds.append(seed_log_weight(net, L, 1.0 / T, 1.95))
             meta.append((L, T))
     net.close()
-    out, gpu = converge(ins, lnw_init=seeds, stratified=False, max_gpu_s=max_gpu_s, chunk=1,
+    out, gpu = converge(ins, lnw_init=seeds, stratified=True, max_gpu_s=max_gpu_s, chunk=1,
                         log=log)
     d = dict(np.load(path))
     added = []
@@ -160,11 +163,16 @@ def extend_corpus(path, Ls, Ts, *, walkers=4096, factor=16.0, max_gpu_s=600.0, l
             d[f"L{L}_T{T}"] = o["lnw"]
             added.append(f"L{L}_T{T}")
     np.savez_compressed(path, **d)
-    return added, gpu
+    return added, gpu, out
 
 
 def load_corpus(path=CORPUS_PATH, exclude=()):
-    """Runs for corpus(): every stored (L, T) once per Delta of the grid."""
+    """Runs for corpus(): every stored (L, T) curve once.  ln W(D) of the
+    D-axis weight problem does not depend on Delta (Delta enters only at
+    reweighting, produce2_multigpu.cu:297), so each sample gets a Delta drawn
+    uniformly from [1.90, 2.00] (fixed generator): the network sees the whole
+    Delta range and learns that the target ignores it, without repeating the
+    curve once per Delta."""
     d = np.load(path)
     runs = []
     for k in sorted(d.files):
@@ -172,8 +180,10 @@ def load_corpus(path=CORPUS_PATH, exclude=()):
             continue
         L = int(k.split("_")[0][1:])
         T = float(k.split("_")[1][1:])
-        for delta in DELTAS:
-            runs.append(dict(L=L, beta=1.0 / T, delta=delta, lnw=d[k]))
+        lnw = d[k]
+        rng = np.random.default_rng(1000 * L + int(round(100 * T)))
+        delta = rng.uniform(DELTAS[0], DELTAS[-1], size=len(lnw))
+        runs.append(dict(L=L, beta=1.0 / T, delta=delta, lnw=lnw))
     return runs
 
 
@@ -200,6 +210,43 @@ def equilibrium_nupdates(L, factor=4):
     larger factor for D_KL < 1e-4 (statistics per bin)."""
     N = L * L
     return int(factor * N * N)
+
+
+def protocol_instances(L, beta, jacks, walkers, mode, *, s0=50.0, growth=2.0, smax=3200.0,
+                       seed0=777):
+    """Instances of one weight problem, one per jack (independent chains:
+    Philox seed seed0 + 1000 j, walker ids 0..walkers-1 in each).
+
+    mode "plain": the reference's defaults (ln W = 0, NUPDATES = 1, the
+    x1.01 schedule once the histogram spans all bins, random start).
+    mode "seeded" / "nonet": NUPDATES = s0 sweeps (s0 N attempts per walker)
+    from the start, multiplied by `growth` each iteration once the histogram
+    spans all bins, up to smax sweeps; "seeded" starts from the surrogate's
+    ln W and "nonet" from ln W = 0 (the same schedule without the network).
+    Both use the stratified start (walkers spread over D)."""
+    N = L * L
+    ins = []
+    for j in range(jacks):
+        q = dict(L=L, walkers=walkers, seed=seed0 + 1000 * j, beta=beta, recursion=False)
+        if mode == "plain":
+            q["init_nup"] = 1
+        else:
+            q.update(init_nup=int(s0 * N), nup_growth=float(growth), nup_max=int(smax * N))
+        ins.append(q)
+    return ins
+
+
+def jackknife_lnw(lnws):
+    """Per-bin jackknife mean and error of J independent ln W curves (each
+    normalised to ln W(0) = 0), as the reference's jakknife
+    (muca_delta_multigpu.cu:620-635) applied bin by bin."""
+    a = np.asarray(lnws, np.float64)
+    J = a.shape[0]
+    tot = a.sum(axis=0)
+    loo = (tot[None, :] - a) / (J - 1)
+    mean = tot / J
+    err = np.sqrt((J - 1) / J * ((loo - mean[None, :]) ** 2).sum(axis=0))
+    return mean, err
 
 
 def converge(instances, *, lnw_init=None, stratified=True, max_gpu_s=60.0, chunk=20,
@@ -251,3 +298,41 @@ def converge(instances, *, lnw_init=None, stratified=True, max_gpu_s=60.0, chunk
         return out, gpu
     finally:
         b.close()
+
+
+def main(argv=None):
+    """python -m muca_amd.surrogate extend --L 72 96 --T 0.5 0.6 0.7 0.8
+    [--corpus PATH] [--walkers 131072] [--s0 50] [--growth 2] [--smax 3200]
+    [--cap 600]: converge new (L, T) curves from surrogate seeds and add the
+    flat ones to the corpus file."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(prog="python -m muca_amd.surrogate")
+    ap.add_argument("command", choices=["extend"])
+    ap.add_argument("--L", type=int, nargs="+", required=True)
+    ap.add_argument("--T", type=float, nargs="+", required=True)
+    ap.add_argument("--corpus", default=CORPUS_PATH)
+    ap.add_argument("--walkers", type=int, default=131072)
+    ap.add_argument("--s0", type=float, default=50.0)
+    ap.add_argument("--growth", type=float, default=2.0)
+    ap.add_argument("--smax", type=float, default=3200.0)
+    ap.add_argument("--cap", type=float, default=600.0)
+    a = ap.parse_args(argv)
+
+    def log(gpu, st):
+        print(f"gpu {gpu:.1f}s " + " ".join(
+            f"[it {x['iter']} w {x['width']} dkl {x['dkl']:.1e} c {x['converged']}]" for x in st),
+            file=sys.stderr, flush=True)
+
+    added, gpu, out = extend_corpus(a.corpus, a.L, a.T, walkers=a.walkers, s0=a.s0,
+                                    growth=a.growth, smax=a.smax, max_gpu_s=a.cap, log=log)
+    print(json.dumps({"added": added, "gpu_s": gpu,
+                      "runs": [{"L": q, "T": T, "flat": bool(o["status"]["converged"]),
+                                "iterations": o["status"]["iter"], "gpu_s": o["gpu_s"],
+                                "last_dkl": o["status"]["dkl"]}
+                               for (q, T), o in zip([(L, T) for L in a.L for T in a.T], out)]}))
+
+
+if __name__ == "__main__":
+    main()

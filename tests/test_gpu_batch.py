@@ -236,3 +236,34 @@ def test_batch_autotune_keeps_weights_set_before():
         b.iterate(1)
         for i in range(b.n):
             assert np.array_equal(b.get_hist(i), got[i])
+
+
+def test_batch_nupdates_growth_and_cap():
+    """A seeded-protocol schedule: once width == N, NUPDATES = (NUPDATES + 1)
+    * growth each iteration, stopping at nup_max; the reference's 1.01 stays
+    the default.  Each iteration's histogram holds walkers x NUPDATES entries."""
+    from math import comb, log
+    # 16 walkers: too few samples for D_KL < 1e-4, so neither converges
+    ins = [dict(L=4, walkers=16, seed=3, init_nup=10, nup_growth=2.0, nup_max=1000, beta=0.01),
+           dict(L=4, walkers=16, seed=4, init_nup=10, beta=0.01)]
+    # near-flat weights at small beta (minus the log of the number of states
+    # with each D): the histogram spans all 17 bins from the first iteration
+    flat = np.array([-(log(comb(16, D)) + D * log(2)) for D in range(17)])
+    with M.Batch(ins, tiles=M.KERNEL_LANE) as b:
+        b.init(0, stratified=True)
+        for i in range(2):
+            b.set_log_weight(i, flat - flat[0])
+        expect = [10, 10]
+        for k in range(12):
+            b.iterate(1)
+            st = b.status()
+            for i in range(2):
+                assert int(b.get_hist(i).sum()) == 16 * expect[i], (k, i)
+                if st[i]["converged"]:
+                    continue
+                if st[i]["width"] == 16:
+                    g = 2.0 if i == 0 else 1.01
+                    expect[i] = min(int((expect[i] + 1) * g), 1000 if i == 0 else 1 << 48)
+                assert st[i]["nup"] == expect[i], (k, i, st[i])
+            assert st[0]["nup_capped"] == (1 if expect[0] == 1000 else 0)
+        assert expect[0] == 1000 and not any(x["converged"] for x in b.status())

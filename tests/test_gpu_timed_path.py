@@ -9,10 +9,10 @@ oracle chain fed the device's weights: the histogram bit for bit, the device
 weight_recursion against the host formula (muca_delta_multigpu.cu:833-859)
 within a few ulps, range, width, D_KL and the NUPDATES schedule.
 
-A second case starts the walkers spread over D (stratified start), so the
-first iterations' histogram ranges exceed 2048 bins and k_batch_update walks
-the recursion in several LDS-staged chunks, carrying the running ln W from
-one chunk to the next.
+A second case starts the walkers spread over D (stratified start) under
+near-flat weights, so the histogram ranges exceed 2048 bins and
+k_batch_update walks the recursion in several LDS-staged chunks, carrying the
+running ln W from one chunk to the next.
 """
 import numpy as np
 import pytest
@@ -23,11 +23,13 @@ from test_gpu_batch import OracleInstance
 pytestmark = pytest.mark.gpu
 
 
-def _check_iterations(orc, q, iters, stratified, min_span=None):
+def _check_iterations(orc, q, iters, stratified, min_span=None, lnw0=None):
     spans = []
     with M.Batch([q]) as b:
         assert b.tiles == M.KERNEL_TEAM
         b.init(0, stratified=stratified)
+        if lnw0 is not None:
+            b.set_log_weight(0, lnw0)
         ref = OracleInstance(orc, q, stratified)
         for k in range(iters):
             before = b.get_log_weight(0)
@@ -60,9 +62,23 @@ def test_bench_c2_workload_every_iteration(orc):
     _check_iterations(orc, q, iters=22, stratified=False)
 
 
+def scaled_weights(L):
+    """An approximately flat-histogram ln W at T = 0.5 for side L: the
+    committed converged L = 64 curve, extensive in N, interpolated in D/N."""
+    from muca_amd import surrogate as S
+    runs = [r for r in S.load_corpus() if r["L"] == 64 and abs(r["beta"] - 2.0) < 1e-9]
+    src = runs[0]["lnw"]
+    n0, N = len(src) - 1, L * L
+    lnw = np.interp(np.arange(N + 1) / N, np.arange(n0 + 1) / n0, src / n0) * N
+    return lnw - lnw[0]
+
+
 def test_recursion_over_several_lds_chunks(orc):
-    """Spread walkers give histogram ranges of ~5000 bins: three chunks of the
-    device recursion per iteration."""
+    """Walkers spread over D (stratified start) under near-flat weights give
+    histogram ranges of ~3000 bins, more than the 2048 recursion steps the
+    device update stages in LDS at a time: the recursion runs in two chunks
+    and carries ln W across the boundary."""
     q = dict(L=72, walkers=4096, seed=99, beta=2.0, recursion=True, init_nup=1)
-    spans = _check_iterations(orc, q, iters=20, stratified=True, min_span=2 * 2048)
+    spans = _check_iterations(orc, q, iters=20, stratified=True, min_span=2048,
+                              lnw0=scaled_weights(72))
     print("ranges", spans)

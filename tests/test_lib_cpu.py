@@ -134,10 +134,14 @@ def test_batch_create_rejects_nupdates_beyond_max_before_any_device_call():
     this runs (and fails with MUCA_E_RANGE) without a GPU."""
     import ctypes
     arr = (M.Instance * 1)()
-    arr[0] = M.Instance(16, 16, 1, 0, M.MAX_NUPDATES + 1, 2.0)
+    arr[0] = M.Instance(16, 16, 1, 0, M.MAX_NUPDATES + 1, 2.0, 0.0, 0)
     h = ctypes.c_void_p()
     rc = M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h))
     assert rc == -6 and not h.value
     assert b"MUCA_MAX_NUPDATES" in M.lib().muca_batch_last_error()
-    arr[0] = M.Instance(7, 16, 1, 0, 1, 2.0)
+    arr[0] = M.Instance(16, 16, 1, 0, 1, 2.0, 0.0, M.MAX_NUPDATES + 1)
+    assert M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h)) == -6
+    arr[0] = M.Instance(16, 16, 1, 0, 1, 2.0, 0.5, 0)
+    assert M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h)) == -1
+    arr[0] = M.Instance(7, 16, 1, 0, 1, 2.0, 0.0, 0)
     assert M.lib().muca_batch_create(0, arr, 1, M.REF32, M.KERNEL_AUTO, ctypes.byref(h)) == -1

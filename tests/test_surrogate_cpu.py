@@ -29,8 +29,12 @@ def test_features_ranges():
 def test_committed_corpus():
     runs = S.load_corpus()
     keys = {(r["L"], round(1 / r["beta"], 3)) for r in runs}
-    assert len(runs) == 4 * len(keys) and len(keys) >= 12
-    for r in runs[::4]:
+    assert len(runs) == len(keys) and len(keys) >= 12
+    for r in runs:
+        # one Delta per sample, spread over the grid's range
+        assert r["delta"].shape == r["lnw"].shape
+        assert 1.90 <= r["delta"].min() and r["delta"].max() <= 2.00
+    for r in runs:
         N = r["L"] ** 2
         lnw = r["lnw"]
         assert lnw.shape == (N + 1,) and lnw[0] == 0.0 and np.all(np.isfinite(lnw))
@@ -52,3 +56,23 @@ def test_init_weight_file(tmp_path):
 def test_equilibrium_nupdates():
     assert S.equilibrium_nupdates(8) == 4 * 64 * 64
     assert S.equilibrium_nupdates(8, 16) == 16 * 64 * 64
+
+
+def test_jackknife_lnw_matches_scalar_jackknife():
+    """Bin by bin, the same numbers as the reference's jakknife
+    (muca_delta_multigpu.cu:620-635) on each bin's J values: the standard
+    error of the mean sqrt(var / J) (sample variance), its closed form."""
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(5, 7))
+    mean, err = S.jackknife_lnw(a)
+    assert np.allclose(mean, a.mean(axis=0))
+    assert np.allclose(err, a.std(axis=0, ddof=1) / np.sqrt(5))
+
+
+def test_protocol_instances():
+    ins = S.protocol_instances(128, 1 / 0.55, 4, 131072, "seeded", s0=50, growth=2.0,
+                               smax=3200)
+    assert [q["seed"] for q in ins] == [777, 1777, 2777, 3777]
+    assert all(q["init_nup"] == 50 * 16384 and q["nup_max"] == 3200 * 16384 for q in ins)
+    plain = S.protocol_instances(128, 1 / 0.55, 4, 131072, "plain")
+    assert all(q["init_nup"] == 1 and "nup_growth" not in q for q in plain)
