@@ -154,6 +154,19 @@ int muca_get_joint(muca_ctx *ctx, uint32_t block, uint64_t *joint, uint64_t nbin
  * takes nupdates < 2^32. */
 int muca_get_moments(muca_ctx *ctx, uint32_t block, uint64_t *moments, uint64_t nwords);
 
+/* No counterpart in the reference: muca_reweight for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * muca_reweight, but no allocation of any kind: its scratch (FP64 per-D
+ * moments, crystal fields, output planes, range flag) is a fixed 32 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends.  MUCA_E_RANGE, before any device work, when 8 * (4 * blocks * (N+1)
+ * + 3 * ndelta * blocks + ndelta) bytes plus alignment exceed it.  All calls
+ * on one GPU share that array (it keeps the last call's contents); calls
+ * that overlap on one GPU, from several threads or contexts, race. */
+int muca_reweight_noalloc(muca_ctx *ctx, const double *deltas, uint32_t ndelta, double *absM,
+                          double *chi, double *binder);
+
 /* ---- host-side weight iteration pieces (O(N), run between launches) ---- */
 /* hist_range (muca_delta_multigpu.cu:573-596) */
 int muca_hist_range(const uint64_t *hist, uint32_t N, int32_t *start, int32_t *end);

@@ -159,6 +159,8 @@ _SIGS = {
     "muca_jackknife": (ctypes.c_int, [_f64p, ctypes.c_uint32, _f64p, _f64p]),
     "muca_reweight": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_uint32, _f64p, _f64p,
                                      _f64p]),
+    "muca_reweight_noalloc": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_uint32, _f64p,
+                                             _f64p, _f64p]),
     "muca_acclog_host": (ctypes.c_float, [ctypes.c_uint32]),
     "muca_acclog_host_range": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _f32p]),
     "muca_acclog_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32,

@@ -32,7 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
 # split into four units so that the units compile in parallel
 LIB_SOURCES = ["muca_team_r7_f32.hip", "muca_team_r7_f64.hip", "muca_team_r10_f32.hip",
                "muca_team_r10_f64.hip", "muca_team.hip", "muca_lane.hip", "muca_kernels.hip",
-               "muca_mlp.hip", "muca_batch.hip", "muca_api.cpp"]
+               "muca_mlp.hip", "muca_batch.hip", "muca_api.cpp", "muca_noalloc.hip"]
 CLI_SOURCES = {"multigpu_weight": "cli_weight.cpp", "produce_gpu": "cli_produce.cpp"}
 
 
