@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--therm-mult", type=int, default=1, help="c5: therm = mult * N")
     ap.add_argument("--nup-mult", type=int, default=1, help="c5: NUPDATES = mult * N")
     ap.add_argument("--blocks", type=int, default=10, help="c5: jackknife blocks")
+    ap.add_argument("--weights", default="", help="c5: .npz of converged ln W (the seeded_jack* "
+                    "arrays a seeded run saves); default: the surrogate seed")
     ap.add_argument("--observables", choices=["moments", "joint"], default="moments",
                     help="c5: per-D moments (default) or the reference's joint H(D,M)")
     ap.add_argument("--mode", choices=["all", "both", "seeded", "plain", "nonet"],
@@ -284,6 +286,8 @@ def main():
     ap.add_argument("--T", type=float, default=0.55)
     ap.add_argument("--delta", type=float, default=1.97)
     ap.add_argument("--jacks", type=int, default=4, help="seeded workload: independent jacks")
+    ap.add_argument("--jack0", type=int, default=0,
+                    help="seeded workload: index of the first jack (seed 777 + 1000 j)")
     ap.add_argument("--nets", type=int, default=2,
                     help="seeded workload: networks (different initial parameters) the "
                          "jacks alternate between")
@@ -502,11 +506,11 @@ def seeded_workload(M, args):
             train_info.append({kk: v for kk, v in info.items() if kk != "loss_curve"})
             print(f"seeded: network {k} trained on {info['samples']} samples in "
                   f"{info['gpu_s']:.2f} GPU s", file=sys.stderr, flush=True)
-        seeds = [nets[j % args.nets] for j in range(J)]
+        seeds = [nets[j % args.nets] for j in range(args.jack0, args.jack0 + J)]
     res = {}
     for mode in modes:
         ins = S.protocol_instances(L, beta, J, W, mode, s0=args.s0, growth=args.growth,
-                                   smax=args.smax)
+                                   smax=args.smax, jack0=args.jack0)
         out, gpu = S.converge(ins, lnw_init=seeds if mode == "seeded" else None,
                               stratified=mode != "plain", max_gpu_s=caps[mode],
                               chunk=1 if mode != "plain" else 50, log=log(mode))
@@ -526,7 +530,8 @@ def seeded_workload(M, args):
 
     sm = {k: summary(v) for k, v in res.items()}
     if args.save_lnw:
-        np.savez_compressed(args.save_lnw, **{f"{k}_jack{j}": o["lnw"] for k, v in res.items()
+        np.savez_compressed(args.save_lnw, **{f"{k}_jack{args.jack0 + j}": o["lnw"]
+                                              for k, v in res.items()
                                               for j, o in enumerate(v["out"])})
     speed = {}
     if "seeded" in sm and sm["seeded"]["flat"]:
@@ -600,17 +605,30 @@ def c5_workload(M, args):
     N = L * L
     beta, delta0, seed = 2.0, 1.98, args.seed
     B = args.blocks
-    X, t, w = S.corpus(S.load_corpus())
-    net, _ = S.train(X, t, w)
-    lnw = S.seed_log_weight(net, L, beta, delta0)
-    net.close()
+    if args.weights:
+        # converged weights: the per-bin mean of the jacks a seeded run saved
+        d = np.load(args.weights)
+        keys = sorted(k for k in d.files if k.startswith("seeded_jack"))
+        lnw_jacks = [d[k] for k in keys]
+        if not lnw_jacks or len(lnw_jacks[0]) != N + 1:
+            raise SystemExit(f"{args.weights}: no seeded_jack* arrays of N+1 = {N + 1} values")
+        lnw, lnw_err = S.jackknife_lnw(lnw_jacks) if len(lnw_jacks) > 1 else (lnw_jacks[0], None)
+        lnw = lnw - lnw[0]
+        weights_note = (f"converged ln W, jackknife mean of {len(lnw_jacks)} jacks "
+                        f"({os.path.basename(args.weights)})")
+    else:
+        X, t, w = S.corpus(S.load_corpus())
+        net, _ = S.train(X, t, w)
+        lnw = S.seed_log_weight(net, L, beta, delta0)
+        net.close()
+        weights_note = "surrogate seed (not converged)"
     therm, nup = args.therm_mult * N, args.nup_mult * N
     print(f"c5: L={L} walkers={W} blocks={B} therm={therm} NUPDATES={nup}", file=sys.stderr,
           flush=True)
     joint = args.observables == "joint"
     obs_mode = M.OBS_JOINT if joint else M.OBS_MOMENTS
     with M.Context(L, W, keying=M.KEYING_PRODUCE, precision=M.FP64, joint_blocks=B,
-                   observables=obs_mode) as ctx:
+                   observables=obs_mode, kernel=args.kernel) as ctx:
         ctx.init_lattice(mm=0, gpu=0, seed=seed)
         ctx.compute_state()
         ctx.set_log_weight(lnw)
@@ -671,7 +689,7 @@ def c5_workload(M, args):
                    "observables": args.observables,
                    "accumulator_words_per_block": (N + 1) * (N + 2) // 2 if joint
                    else 8 * (N + 1),
-                   "weights": "surrogate seed (not converged)"},
+                   "weights": weights_note},
         "kernel": {1: "generic", 2: "team", 3: "lane"}.get(kernel_used, "?"),
         "block": block_used,
         "reweight_ms": rw_ms,

@@ -67,7 +67,8 @@ typedef struct muca_params {
                             walkers and compute units; else 64..1024) */
     int32_t observables; /* produce path with joint_blocks > 0: MUCA_OBS_JOINT keeps
                             the reference's joint H(D,M); MUCA_OBS_MOMENTS keeps
-                            exact per-D moments instead (6 (N+1) words per block) */
+                            exact per-D moments instead (6 (N+1) words per block;
+                            MUCA_FP64, generic or team kernel) */
 } muca_params;
 
 typedef struct muca_info {

@@ -53,16 +53,17 @@ hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a);
 // team kernel (muca_team.hip): LDS layout of one workgroup of 16 walkers
 struct TeamLayout {
     uint32_t row;         // L/2 attempts per half-row
-    uint32_t team_bytes;  // per-team Philox ring and attempt records
-    uint32_t off_rec;     // offset of the attempt records inside a team's area
+    uint32_t team_bytes;  // per-team Philox ring and ln W window
+    uint32_t off_fde;     // (T)(-dE * beta) for dE = -8 .. 8
     uint32_t ring;        // Philox blocks per key held in a team's ring
+    uint32_t off_win;     // inside a team's area: its ln W window (ln W in HBM)
     uint32_t off_team, off_tab, off_w, off_hist, off_lat;
     uint32_t lat_stride;  // LDS bytes per walker lattice (>= N, bank-spread)
     size_t bytes;         // dynamic LDS per workgroup
     bool lds_hist, lat_lds;
+    bool lnw_global;      // ln W too large for LDS: read from HBM (L2) per window
 };
-// false when the team kernel cannot run this launch (L % 8 != 0, L > 256,
-// or ln W does not fit in LDS)
+// false when the team kernel cannot run this launch (L % 8 != 0 or L > 256)
 bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay);
 hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
                            const TeamLayout &lay);
@@ -173,6 +174,10 @@ hipError_t launch_joint_moments(hipStream_t st, const uint64_t *joint, uint32_t 
                                 uint64_t stride, uint32_t blocks, double *mom);
 // p[0..n) = v (a kernel, so that callers may capture it in a graph)
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+// D histogram of a launch that kept per-D moments: hist[D] = sum over the
+// blocks of S0[D]
+hipError_t launch_hist_from_moments(hipStream_t st, const uint64_t *moms, uint32_t N,
+                                    uint32_t blocks, uint64_t *hist);
 // reweighting of per-D moments at ndelta crystal fields; out holds three
 // planes of ndelta*blocks values (<|M|>, chi, Binder), [delta][block] each
 hipError_t launch_reweight(hipStream_t st, const double *mom, const double *lnw, uint32_t N,

@@ -105,8 +105,8 @@ int muca_create(const muca_params *params, muca_ctx **out)
     const bool moments = p.joint_blocks > 0 && p.observables == MUCA_OBS_MOMENTS;
     if (moments && p.precision != MUCA_FP64)
         return fail(MUCA_E_ARG, "per-D moments are accumulated in MUCA_FP64 mode only");
-    if (moments && p.kernel != MUCA_KERNEL_AUTO && p.kernel != MUCA_KERNEL_GENERIC)
-        return fail(MUCA_E_ARG, "per-D moments are accumulated by the generic kernel only");
+    if (moments && p.kernel == MUCA_KERNEL_LANE)
+        return fail(MUCA_E_ARG, "per-D moments are accumulated by the generic and team kernels only");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (p.device < 0 || p.device >= ndev)
@@ -155,7 +155,8 @@ int muca_create(const muca_params *params, muca_ctx **out)
         const bool team_ok = muca::team_layout(probe, p.precision == MUCA_FP64, lds_probe, lay_probe);
         const bool lane_ok = muca::lane_eligible(probe, p.precision == MUCA_FP64, lds_probe);
         if (moments)
-            c->cfg.kernel = MUCA_KERNEL_GENERIC;
+            c->cfg.kernel = team_ok && (uint64_t)W <= 128ull * ncu ? MUCA_KERNEL_TEAM
+                                                                   : MUCA_KERNEL_GENERIC;
         else if (team_ok && (uint64_t)W <= 128ull * ncu)
             c->cfg.kernel = MUCA_KERNEL_TEAM;
         else if (lane_ok)
@@ -480,8 +481,19 @@ int muca_autotune(muca_ctx *c, uint64_t attempts, int32_t *kernel, uint32_t *blo
     };
     std::vector<Cand> cands;
     muca::TeamLayout lay{};
-    const bool moments = c->moms != nullptr;   // accumulated by the generic kernel only
-    if (!moments && muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay))
+    const bool moments = c->moms != nullptr;   // accumulated by the generic and team kernels
+    if (moments) {
+        // timed as production runs: with the per-D moments
+        if (a.nup >> 32) {
+            hipFree(lat0);
+            hipFree(st0);
+            return fail(MUCA_E_ARG, "per-D moments take nupdates < 2^32 per launch");
+        }
+        a.moms = c->moms;
+        a.joint_blocks = c->p.joint_blocks;
+        a.block_walkers = c->block_walkers ? c->block_walkers : c->W;
+    }
+    if (muca::team_layout(a, c->cfg.fp64, c->cfg.lds_max, lay))
         cands.push_back({MUCA_KERNEL_TEAM, 256});
     if (!moments && muca::lane_eligible(a, c->cfg.fp64, c->cfg.lds_max))
         for (int b : {256, 512, 1024}) cands.push_back({MUCA_KERNEL_LANE, b});
@@ -497,6 +509,10 @@ int muca_autotune(muca_ctx *c, uint64_t attempts, int32_t *kernel, uint32_t *blo
         for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
             e = save_restore(false);
             if (e == hipSuccess) e = hipMemsetAsync(c->hist, 0, (N + 1) * 8, c->st);
+            if (e == hipSuccess && moments)
+                e = hipMemsetAsync(c->moms, 0,
+                                   (uint64_t)muca::kMomWords * (N + 1) * c->p.joint_blocks * 8,
+                                   c->st);
             if (e == hipSuccess) e = hipEventRecord(c->ev0, c->st);
             if (e == hipSuccess) e = muca::launch_mcs(c->st, cfg, a);
             if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);

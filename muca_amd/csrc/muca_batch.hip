@@ -427,7 +427,8 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
         a.L = I.L;
         a.W = I.W;
         a.nup = 1;
-        if (!(I.L % 8 == 0 && I.L <= 128 && muca::team_layout(a, b->fp64, lds_budget, I.lay)))
+        if (!(I.L % 8 == 0 && I.L <= 128 && muca::team_layout(a, b->fp64, lds_budget, I.lay) &&
+              !I.lay.lnw_global))
             team_ok = false;
         const size_t tsz = b->fp64 ? 8 : 4;
         I.lane_lds_w = (uint32_t)(((I.N + 1) * tsz + 15) & ~(size_t)15);

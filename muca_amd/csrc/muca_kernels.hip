@@ -8,6 +8,7 @@
 // in LDS (uint32) and flushed to HBM with 64-bit atomics.  Nothing in the
 // update is a dense contraction, so there is no MFMA here.
 #include "kernels.h"
+#include "moments.h"
 
 #include "acclog.h"
 #include "philox.h"
@@ -137,50 +138,6 @@ template <>
 __device__ __forceinline__ double lnw_at<double>(const McsArgs &a, int32_t i)
 {
     return a.lnw64[i];
-}
-
-// A 128-bit value (vl, vh < 2^32) added to three limb words w0 + w1 2^32 +
-// w2 2^64 with non-returning atomics (no carry to detect: each word gains
-// less than 2^32 per add).
-__device__ __forceinline__ void atomic_add_limbs(uint64_t *w0, uint64_t *w1, uint64_t *w2,
-                                                 uint64_t vl, uint64_t vh)
-{
-    const uint64_t l0 = vl & 0xFFFFFFFFull, l1 = vl >> 32;
-    if (l0) atomicAdd((unsigned long long *)w0, (unsigned long long)l0);
-    if (l1) atomicAdd((unsigned long long *)w1, (unsigned long long)l1);
-    if (vh) atomicAdd((unsigned long long *)w2, (unsigned long long)vh);
-}
-
-// A walker's production attempts at one D, summed in registers and added to
-// its block's per-D moments when D changes: count, sum |M|, and 128-bit sums
-// of M^2 and M^4 (|M| <= N <= 2^16, so M^4 <= 2^64 needs a high word too).
-struct MomRun {
-    uint64_t n, s1, s2l, s2h, s4l, s4h;
-    int32_t d;
-};
-
-__device__ __forceinline__ void mom_add(MomRun &r, int32_t m)
-{
-    const uint64_t am = (uint64_t)(m < 0 ? -m : m);
-    const uint64_t m2 = am * am;
-    const uint64_t m4l = m2 * m2, m4h = __umul64hi(m2, m2);
-    r.n += 1;
-    r.s1 += am;
-    r.s2l += m2;
-    r.s2h += r.s2l < m2 ? 1u : 0u;
-    r.s4l += m4l;
-    r.s4h += m4h + (r.s4l < m4l ? 1u : 0u);
-}
-
-__device__ __forceinline__ void mom_flush(const MomRun &r, uint64_t *mb, uint64_t nb)
-{
-    if (r.n == 0) return;
-    const uint64_t d = (uint64_t)r.d;
-    atomicAdd((unsigned long long *)&mb[d], (unsigned long long)r.n);
-    atomicAdd((unsigned long long *)&mb[nb + d], (unsigned long long)r.s1);
-    // a run holds at most NUPDATES < 2^32 attempts, so the high words are < 2^32
-    atomic_add_limbs(&mb[2 * nb + d], &mb[3 * nb + d], &mb[4 * nb + d], r.s2l, r.s2h);
-    atomic_add_limbs(&mb[5 * nb + d], &mb[6 * nb + d], &mb[7 * nb + d], r.s4l, r.s4h);
 }
 
 // n entries of D histogram bin d: LDS (uint32, a workgroup adds at most
@@ -448,11 +405,17 @@ static hipError_t launch_generic_r(hipStream_t st, const Launch &cfg, const McsA
         launch_generic_h<R, FP64, KEYING, true>(st, grid, block, lds, a);
     else
         launch_generic_h<R, FP64, KEYING, false>(st, grid, block, lds, a);
-    if (FP64 && a.moms && !a.joint && a.nup > 0) {
-        const uint64_t nb = (uint64_t)a.N + 1;
-        hipLaunchKernelGGL(k_hist_from_moments, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
-                           st, (const uint64_t *)a.moms, a.N, a.joint_blocks, a.hist);
-    }
+    if (FP64 && a.moms && !a.joint && a.nup > 0)
+        return launch_hist_from_moments(st, a.moms, a.N, a.joint_blocks, a.hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_from_moments(hipStream_t st, const uint64_t *moms, uint32_t N,
+                                    uint32_t blocks, uint64_t *hist)
+{
+    const uint64_t nb = (uint64_t)N + 1;
+    hipLaunchKernelGGL(k_hist_from_moments, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
+                       moms, N, blocks, hist);
     return hipGetLastError();
 }
 
@@ -466,9 +429,9 @@ static hipError_t launch_generic_k(hipStream_t st, const Launch &cfg, const McsA
 hipError_t launch_mcs(hipStream_t st, const Launch &cfg, const McsArgs &a)
 {
     if (a.W == 0 || (a.therm == 0 && a.nup == 0)) return hipSuccess;
-    // per-D moment accumulation exists in the generic kernel only
+    // per-D moments: the generic kernel, or (FP64) the team kernel
     const bool moments = a.moms != nullptr && a.joint == nullptr;
-    if (cfg.kernel == 2 && !moments) {
+    if (cfg.kernel == 2 && (!moments || cfg.fp64)) {
         TeamLayout lay{};
         if (team_layout(a, cfg.fp64, cfg.lds_max, lay)) return launch_mcs_team(st, cfg, a, lay);
     }

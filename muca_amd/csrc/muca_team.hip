@@ -24,21 +24,26 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     if (a.L % 8 != 0 || a.L > 256) return false;
     const size_t tsz = fp64 ? 8 : 4;
     const size_t row = a.L / 2;
-    const size_t recsz = fp64 ? sizeof(TeamRec<double>) : sizeof(TeamRec<float>);
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     lay.row = (uint32_t)row;
     // Philox ring: a half-row needs row/4 blocks per key and a generation pass
-    // may run up to 7 blocks ahead of them
+    // may run up to 7 blocks ahead of them; then the team's ln W window
     lay.ring = (uint32_t)(row / 4 + 8);
-    lay.off_rec = (uint32_t)al(2 * (size_t)lay.ring * 16);
-    lay.team_bytes = (uint32_t)(lay.off_rec + al(row * recsz));
+    lay.off_win = (uint32_t)al(2 * (size_t)lay.ring * 16);
+    lay.team_bytes = (uint32_t)(lay.off_win + al((kWin + 2) * tsz));
     size_t off = 0;
     lay.off_team = 0;
     off += al((size_t)lay.team_bytes * kTeams);
     lay.off_tab = (uint32_t)off;
     off += 256 * sizeof(double);
+    lay.off_fde = (uint32_t)off;
+    off += al(17 * tsz);
     lay.off_w = (uint32_t)off;
-    off += al((a.N + 1 + 2 * kGuard) * tsz);
+    // ln W in LDS when it fits (with the guards), else read from HBM, where it
+    // stays in L2 (512 KB at L = 256 in FP64), one window per team at a time
+    const size_t wbytes = al((a.N + 1 + 2 * kGuard) * tsz);
+    lay.lnw_global = off + wbytes > lds_max;
+    if (!lay.lnw_global) off += wbytes;
     // histogram and lattice in LDS when they fit (histogram first)
     const bool hist_fits = off + al((a.N + 1) * 4) <= lds_max;
     lay.lds_hist = hist_fits;
@@ -99,12 +104,12 @@ __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
         const TeamLayout lay = I->lay;
         const uint32_t nr = (lay.row + 15) / 16;
         const int v = (lay.lat_lds ? 1 : 0) | (lay.lds_hist ? 2 : 0);
-#define MUCA_BT(NR)                                                                         \
-    switch (v) {                                                                            \
-    case 0: team_run<7, FP64, 0, false, false, false, NR>(a, lay, tile.wbase, lds); break;  \
-    case 1: team_run<7, FP64, 0, true, false, false, NR>(a, lay, tile.wbase, lds); break;   \
-    case 2: team_run<7, FP64, 0, false, true, false, NR>(a, lay, tile.wbase, lds); break;   \
-    default: team_run<7, FP64, 0, true, true, false, NR>(a, lay, tile.wbase, lds); break;   \
+#define MUCA_BT(NR)                                                                          \
+    switch (v) {                                                                             \
+    case 0: team_run<7, FP64, 0, false, false, 0, false, NR>(a, lay, tile.wbase, lds); break; \
+    case 1: team_run<7, FP64, 0, true, false, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
+    case 2: team_run<7, FP64, 0, false, true, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
+    default: team_run<7, FP64, 0, true, true, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
     }
         if (nr <= 1) {
             MUCA_BT(1)

@@ -9,15 +9,15 @@
 // neighbour of a half-row was last written by an earlier half-row.  So per
 // half-row the team
 //   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
-//      the proposed spin, dE, dDel, (T)(-dE*beta), the acceptance log and an
-//      acceptance bitmask over a window of kWin D values around the D the
-//      row starts from;
+//      the proposed spin, dE, dDel, (T)(-dE*beta) and an acceptance bitmask
+//      over a window of kWin D values around the D the row starts from,
+//      screened with the hardware log (the exact log only where the screen
+//      is inconclusive);
 //   B. runs only the D chain serially: attempt j's mask is broadcast from
 //      its lane with a DPP row broadcast and the accept bit is read at the
 //      current D -- no memory access and no branch on the chain.  A round of
 //      16 steps in which D left the window (more than kWin/2 net moves from
-//      the row's start) is replayed with an exact per-step slow path from
-//      LDS;
+//      the row's start) is replayed with an exact per-step slow path;
 //   C. writes the accepted spins, sums dE and dM and fills the histogram.
 // Every acceptance is computed with the reference's operations and rounding:
 //   diff = ((T)(-dE*beta) + lnW[D+dDel]) - lnW[D];  accept iff diff >= 0 or
@@ -25,14 +25,15 @@
 //   acclog(r1) < diff.  Results are bit-identical to the one-lane kernel.
 // Requires L % 8 == 0 (whole Philox blocks per half-row) and L <= 256.
 //
-// LDS (dynamic): per team a ring of Philox blocks [2 keys][L/8 + 8][4] and
-// attempt records [L/2] (slow path); the acceptance-log table; ln W with guard entries on
-// each side; the D histogram (u32, when it fits); the workgroup's lattices
+// LDS (dynamic): per team a ring of Philox blocks [2 keys][L/8 + 8][4]; the
+// acceptance-log table; (T)(-dE*beta) for dE = -8..8; ln W with guard entries
+// on each side; the D histogram (u32, when it fits); the workgroup's lattices
 // [N][16] int8 (when they fit).
 #pragma once
 #include "kernels.h"
 
 #include "acclog.h"
+#include "moments.h"
 #include "philox.h"
 
 namespace muca {
@@ -50,13 +51,6 @@ constexpr int kTeams = 16;   // per workgroup of 256 lanes
 constexpr int kWin = 8;
 constexpr int kGuard = 24;   // ln W guard entries on each side (> kWin / 2 + 2)
 
-template <typename T>
-struct TeamRec {
-    T fde;          // (T)(-dE * beta)
-    float logu;     // acceptance log of r1
-    int32_t ddel;   // new^2 - old^2
-};
-
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_wave_barrier();
@@ -67,6 +61,51 @@ __device__ __forceinline__ int32_t team_sum(int32_t v)
 #pragma unroll
     for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, kTeam);
     return v;
+}
+
+// the word of x that holds its sign bit (bit 31)
+__device__ __forceinline__ uint32_t sign_word(float x)
+{
+    return __float_as_uint(x);
+}
+__device__ __forceinline__ uint32_t sign_word(double x)
+{
+    return (uint32_t)(__double_as_longlong(x) >> 32);
+}
+
+// value of lane (i - K) mod 16 of this lane's 16-lane row (v_mov_b32_dpp row_ror:K)
+template <int K>
+__device__ __forceinline__ uint32_t row_ror(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int32_t)v, 0x120 + K, 0xf, 0xf, false);
+}
+
+// sum over the 16 lanes of this lane's row, in every lane of it
+__device__ __forceinline__ uint32_t row_allsum(uint32_t v)
+{
+    v += row_ror<8>(v);
+    v += row_ror<4>(v);
+    v += row_ror<2>(v);
+    v += row_ror<1>(v);
+    return v;
+}
+
+// the same for the 96-bit value lo + hi 2^64
+__device__ __forceinline__ void row_allsum96(uint64_t &lo, uint32_t &hi)
+{
+    auto step = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const uint64_t o = (uint64_t)row_ror<K>((uint32_t)lo) |
+                           ((uint64_t)row_ror<K>((uint32_t)(lo >> 32)) << 32);
+        const uint32_t oh = row_ror<K>(hi);
+        const uint64_t s = lo + o;
+        hi = hi + oh + (s < lo ? 1u : 0u);
+        lo = s;
+    };
+    step(std::integral_constant<int, 8>());
+    step(std::integral_constant<int, 4>());
+    step(std::integral_constant<int, 2>());
+    step(std::integral_constant<int, 1>());
 }
 
 // value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L;
@@ -81,36 +120,53 @@ __device__ __forceinline__ int32_t row_bcast(int32_t v)
 using namespace team_detail;
 
 // NR: rounds of 16 attempts per half-row, NR * 16 >= L / 2
+// OBS: 0 the D histogram; 1 joint H(D,M) per walker block (produce2_multigpu.cu:489);
+// 2 exact per-D moments per walker block (the D histogram is then their S0).
+// LNWG: ln W read from HBM (it does not fit in LDS), one window per team.
 // One workgroup's sixteen walkers wbase .. wbase+15 of the launch described
 // by a (all 256 lanes of the workgroup call it together).
-template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
+template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, int OBS, bool LNWG, int NR>
 __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay, uint64_t wbase,
                                          unsigned char *lds)
 {
     typedef typename std::conditional<FP64, double, float>::type T;
+    constexpr bool JOINT = OBS == 1, MOMS = OBS == 2, TRACKM = OBS >= 1;
     __syncthreads();   // LDS may still be read by a previous call's flush
     T *wt = (T *)(lds + lay.off_w) + kGuard;   // valid for [-kGuard, N + kGuard]
     uint32_t *hist = (uint32_t *)(lds + lay.off_hist);
     int8_t *latl = (int8_t *)(lds + lay.off_lat);
     double *logtab = (double *)(lds + lay.off_tab);
+    T *fdet = (T *)(lds + lay.off_fde) + 8;    // fdet[dE] = (T)(-dE * beta), |dE| <= 8
     const int tid = threadIdx.x, team = tid >> 4, tl = tid & 15;
     const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
     unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
     uint32_t *rng = (uint32_t *)tbase;                          // [2][ring][4]
-    TeamRec<T> *rec = (TeamRec<T> *)(tbase + lay.off_rec);      // [Rw]
+    T *winb = (T *)(tbase + lay.off_win);                       // LNWG: lnW[dlo - 1 .. dlo + kWin]
+    // ln W at any D: from LDS, or from HBM (clamped into [0, N]; only
+    // positions a walker cannot be at read a clamped value)
+    auto lnw_at = [&](int32_t i) -> T {
+        if (LNWG) {
+            const int32_t c = min(max(i, 0), (int32_t)a.N);
+            return FP64 ? (T)a.lnw64[c] : (T)a.lnw32[c];
+        }
+        return wt[i];
+    };
 
     const uint64_t W = a.W;
     const uint64_t w = wbase + team;
     const bool active = w < W;
 
-    for (int i = (int)tid - kGuard; i <= (int)N + kGuard; i += blockDim.x) {
-        T v = (T)0;
-        if (i >= 0 && i <= (int)N) v = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
-        wt[i] = v;
+    if (!LNWG) {
+        for (int i = (int)tid - kGuard; i <= (int)N + kGuard; i += blockDim.x) {
+            T v = (T)0;
+            if (i >= 0 && i <= (int)N) v = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
+            wt[i] = v;
+        }
     }
     if (LDSHIST)
         for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = a.hist_preload;
     acclog_table_to_lds(logtab, tid, blockDim.x);
+    if (tid < 17) fdet[tid - 8] = (T)(-(double)(tid - 8) * a.beta);
     if (LATLDS) {
         for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
             const uint32_t site = e >> 4, t = e & 15;
@@ -132,6 +188,16 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         int32_t d = a.D[w], e = a.E[w], m = a.M[w];
         uint64_t *jbase = nullptr;
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
+        // MOMS: the team's production attempts at one D, summed over its lanes
+        // round by round and kept for the D in use and the one before it (the
+        // same in every lane of the team), added to the block's moments when a
+        // third D comes up
+        const uint64_t mstride = (uint64_t)N + 1;
+        uint64_t *mbase = nullptr;
+        if (MOMS) mbase = a.moms + (w / a.block_walkers) * (uint64_t)kMomWords * mstride;
+        MomRun mrun{}, mprev{};
+        mrun.d = -1;
+        mprev.d = -1;
         // joint-histogram entries of this lane's attempts, counted for the bin
         // in use and the one before it and added with one atomic when a third
         // bin comes up (a walker stepping back and forth adds nothing)
@@ -179,13 +245,66 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 cur_slot = cur_slot + nblk >= RB ? cur_slot + nblk - RB : cur_slot + nblk;
                 wave_sync();
                 // ---- A2: attempt j = 16 r + tl on this lane, everything in registers
-                const int32_t dlo = d - kWin / 2;     // mask bit k <-> D = dlo + k
-                T win[kWin + 2];                      // lnW[dlo - 1 .. dlo + kWin]
+                int32_t dlo = d - kWin / 2;           // mask bit k <-> D = dlo + k
+                // lnW[dlo + k] at wcen[k], k = -1 .. kWin (LNWG: staged in the
+                // team's window from HBM)
+                auto window = [&]() -> const T * {
+                    if (LNWG) {
+                        if (tl < kWin + 2) winb[tl] = lnw_at(dlo - 1 + tl);
+                        wave_sync();
+                        return winb + 1;
+                    }
+                    return wt + dlo;
+                };
+                const T *wcen = window();
+                T wfrom[kWin];                        // lnW[dlo .. dlo + kWin - 1]
 #pragma unroll
-                for (int k = 0; k < kWin + 2; k++) win[k] = wt[dlo - 1 + k];
+                for (int k = 0; k < kWin; k++) wfrom[k] = wcen[k];
+                const int32_t dlo_a = dlo;            // the window of every mask made here
                 const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
+                const uint32_t s4 = slot0 * 4, ringw = RB * 4;   // ring word of attempt j: s4 + j (mod ringw)
                 uint32_t mask[NR];
                 int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
+                // Bits k = 0..kWin-1: accept at D = dlo + k, i.e. acclog(r1) <
+                // ((fde + lnW[D + dDel]) - lnW[D]), screened with the hardware
+                // log and its proven band: la + band < diff accepts, la - band
+                // >= diff rejects.  Bit k is the sign of e = thi - diff (e is
+                // -0 never, +0 iff thi == diff, which rejects); a diff with
+                // tlo < diff <= thi, i.e. 0 <= e < thi - tlo, needs the exact
+                // log -- flagged (with margin) by min |e| < 2 (thi - tlo).
+                auto accept_bits = [&](const T *wf, const T *wc, uint32_t r1, T fde,
+                                       int32_t ddel) -> uint32_t {
+                    const T *wto = wc + ddel;
+                    T diff[kWin];
+#pragma unroll
+                    for (int k = 0; k < kWin; k++) {
+                        diff[k] = fde + wto[k];
+                        diff[k] = diff[k] - wf[k];
+                    }
+                    const float la = acclog_fast_u01(r1);
+                    const float band = acclog_fast_band(la);
+                    const T thi = (T)(la + band), tlo = (T)(la - band);
+                    uint32_t mk = 0;
+                    T emin = (T)1e30f;
+#pragma unroll
+                    for (int k = kWin - 1; k >= 0; k--) {
+                        const T e = thi - diff[k];
+                        mk = __builtin_amdgcn_alignbit(mk, sign_word(e), 31);
+                        emin = fmin(emin, fabs(e));
+                    }
+                    const bool amb = emin < (thi - tlo) + (thi - tlo);
+                    if (__builtin_amdgcn_ballot_w64(amb)) {   // rare: a diff within the band
+                        if (amb) {
+                            const T tlog =
+                                (T)(float)acclog_u01_double_tab(r1, logtab, logtab + 128);
+                            mk = 0;
+#pragma unroll
+                            for (int k = kWin - 1; k >= 0; k--)
+                                mk = mk + mk + (tlog < diff[k] ? 1u : 0u);
+                        }
+                    }
+                    return mk;
+                };
                 // Branch-free over the rounds so that the loads of all rounds
                 // can be in flight together: a lane past the end of the
                 // half-row works on a clamped (in-bounds) attempt and keeps
@@ -201,43 +320,22 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t old_s = sp(y * L + x);
                     const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
                                        sp(yd * L + x);
-                    uint32_t rslot = slot0 + (jc >> 2);
-                    rslot = rslot >= RB ? rslot - RB : rslot;
-                    const uint32_t widx = rslot * 4 + (jc & 3);
-                    const uint32_t r1 = rng[widx], r3 = rng[RB * 4 + widx];
+                    uint32_t widx = s4 + jc;
+                    widx = widx >= ringw ? widx - ringw : widx;
+                    const uint32_t r1 = rng[widx], r3 = rng[ringw + widx];
                     int32_t new_s = old_s + (r3 < 0x80000000u ? 1 : -1);
                     new_s = new_s == 2 ? -1 : (new_s == -2 ? 1 : new_s);
                     const int32_t ds = new_s - old_s;
                     const int32_t dE = -ds * nb;
                     const int32_t ddel = new_s * new_s - old_s * old_s;
-                    const T fde = (T)(-(double)dE * a.beta);
-                    // the correctly rounded acceptance log (a kWin-wide mask needs it
-                    // for every attempt; screening each of the 16 comparisons with
-                    // the hardware log costs more than it saves here)
-                    const float logu = (float)acclog_u01_double_tab(r1, logtab, logtab + 128);
-                    const T tlog = (T)logu;
-                    uint32_t mk = 0;
-#pragma unroll
-                    for (int k = kWin - 1; k >= 0; k--) {
-                        const T wa = ddel > 0 ? win[k + 2] : (ddel < 0 ? win[k] : win[k + 1]);
-                        T diff = fde + wa;
-                        diff = diff - win[k + 1];
-                        mk = mk + mk + (tlog < diff ? 1u : 0u);
-                    }
-                    // bits 0..15: accept at D = dlo + k; bits 16..17: dDel
+                    const uint32_t mk = accept_bits(wfrom, wcen, r1, fdet[dE], ddel);
+                    // bits 0..kWin-1: accept at D = dlo + k; bits 16..17: dDel
                     mask[r] = mk | (valid ? ((uint32_t)ddel & 3u) << 16 : 0u);
                     if (JOINT)
                         info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
                                         : ((8 << 2) | (2 << 7));
                     else
                         info[r] = valid ? ((dE + 8) << 2) | ((ds + 2) << 7) : ((8 << 2) | (2 << 7));
-                    if (valid) {
-                        TeamRec<T> rc;
-                        rc.fde = fde;
-                        rc.logu = logu;
-                        rc.ddel = ddel;
-                        rec[j] = rc;
-                    }
                 }
                 wave_sync();
                 // ---- B: the D chain.  The chain runs on `off` = D - dlo (the
@@ -273,14 +371,30 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     // v_bfe_u32 reads the offset modulo 32
                     uint32_t acc = __builtin_amdgcn_ubfe(mj, (uint32_t)off, 1);
                     if (CHK) {
+                        // a step outside the window: the attempt's dE from its
+                        // info word, its r1 from the ring, the screened log
+                        // (exact only inside the band)
                         const bool oob = (uint32_t)off >= (uint32_t)kWin;
                         if (__builtin_amdgcn_ballot_w64(oob)) {
                             if (oob) {
-                                const TeamRec<T> rc = rec[16 * r + l];
+                                const int32_t ij = row_bcast<l>(info[r]);
+                                uint32_t w1 = s4 + 16 * r + l;
+                                w1 = w1 >= ringw ? w1 - ringw : w1;
+                                const uint32_t r1 = rng[w1];
                                 const int32_t dd = dlo + off;
-                                T diff = rc.fde + wt[dd + rc.ddel];
-                                diff = diff - wt[dd];
-                                acc = (T)rc.logu < diff ? 1u : 0u;
+                                T diff = fdet[((ij >> 2) & 31) - 8] + lnw_at(dd + ddel);
+                                diff = diff - lnw_at(dd);
+                                const float la = acclog_fast_u01(r1);
+                                const float band = acclog_fast_band(la);
+                                acc = (T)(la + band) < diff ? 1u : 0u;
+                                const bool amb = !acc && (T)(la - band) < diff;
+                                if (__builtin_amdgcn_ballot_w64(amb)) {
+                                    if (amb)
+                                        acc = (T)(float)acclog_u01_double_tab(r1, logtab,
+                                                                              logtab + 128) < diff
+                                                  ? 1u
+                                                  : 0u;
+                                }
                             }
                         }
                     }
@@ -332,14 +446,38 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const uint32_t j0 = 16 * r;
                     if (j0 >= cnt) continue;
                     const uint32_t left = cnt - j0;
+                    // the window follows D from round to round: a round that
+                    // starts outside the masks' window gets its masks
+                    // recomputed around the current D, by all lanes of the
+                    // team at once (only this round's: the later rounds'
+                    // masks stay relative to dlo_a)
+                    dlo = dlo_a;
+                    off = d - dlo_a;
+                    const bool shift = (uint32_t)off >= (uint32_t)kWin;
+                    if (__builtin_amdgcn_ballot_w64(shift)) {
+                        if (shift) {
+                            dlo += off - kWin / 2;
+                            off = kWin / 2;
+                            const T *wc = window();
+                            T wf[kWin];
+#pragma unroll
+                            for (int k = 0; k < kWin; k++) wf[k] = wc[k];
+                            uint32_t w1 = s4 + min(16 * r + tl, cnt - 1);
+                            w1 = w1 >= ringw ? w1 - ringw : w1;
+                            const int32_t dd = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
+                            const uint32_t mk = accept_bits(wf, wc, rng[w1],
+                                                            fdet[((info[r] >> 2) & 31) - 8], dd);
+                            mask[r] = mk | (mask[r] & 0x30000u);
+                        }
+                    }
                     const int32_t off0 = off;
                     uint32_t ab = 0;
                     MUCA_STEPS(Fast, ab)
                     int32_t o = recover(r, off0, ab);
-                    // a step started outside the window iff the round did or some
-                    // step of it ended there (a lane past the half-row's end
-                    // repeats the last step's D)
-                    const bool out = (uint32_t)off0 >= (uint32_t)kWin || (uint32_t)o >= (uint32_t)kWin;
+                    // a step started outside the window iff one before it in the
+                    // round ended there (a lane past the half-row's end repeats
+                    // the last step's D)
+                    const bool out = (uint32_t)o >= (uint32_t)kWin && tl + 1 < min(left, 16u);
                     if (__builtin_amdgcn_ballot_w64(out)) {   // rare: replay exactly
                         const uint64_t mine = __builtin_amdgcn_ballot_w64(out) >> sh;
                         if (mine & 0xffffu) {
@@ -352,7 +490,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     accb[r] = ab;
                     dv[r] = dlo + o;
                     d = dlo + off;
-                    if (JOINT) {
+                    if (TRACKM) {
                         // M after this lane's step: an inclusive scan over the
                         // team of the accepted spin changes
                         int32_t x = (int32_t)((ab >> tl) & 1u) * (((info[r] >> 7) & 7) - 2);
@@ -378,7 +516,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                             de += ((info[r] >> 2) & 31) - 8;
                             dm += ds;
                         }
-                        if (phase) {
+                        if (phase && !MOMS) {
                             if (LDSHIST) {
                                 if (may_wrap) {
                                     // a u32 bin that wraps carries 2^32 into HBM
@@ -415,6 +553,59 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         }
                     }
                 }
+                if (MOMS && phase) {
+                    // the round's attempts grouped by D (a round visits a few
+                    // D values), each group summed over the team's lanes and
+                    // added to the team's two-entry cache
+#pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        if (16 * r >= cnt) break;
+                        const int32_t dd = dv[r];
+                        const int32_t mm = min(max(mv[r], -dd), dd);
+                        const uint32_t am = (uint32_t)(mm < 0 ? -mm : mm);
+                        const uint64_t m2 = (uint64_t)am * am;
+                        const uint64_t m4 = m2 * m2;
+                        const uint32_t m4h = (uint32_t)__umul64hi(m2, m2);
+                        uint32_t rem =
+                            (uint32_t)(__builtin_amdgcn_ballot_w64(16 * r + tl < cnt) >> sh) &
+                            0xffffu;
+                        // rem is the same in the sixteen lanes of a team
+                        while (__builtin_amdgcn_ballot_w64(rem != 0)) {
+                            if (rem) {
+                                const int32_t key = __shfl(dd, __builtin_ctz(rem), kTeam);
+                                const bool same = ((rem >> tl) & 1u) && dd == key;
+                                rem &= ~((uint32_t)(__builtin_amdgcn_ballot_w64(same) >> sh) &
+                                         0xffffu);
+                                const uint32_t cn = row_allsum(same ? 1u : 0u);
+                                const uint32_t c1 = row_allsum(same ? am : 0u);
+                                uint64_t c2 = same ? m2 : 0;
+                                uint32_t c2h = 0;
+                                row_allsum96(c2, c2h);
+                                uint64_t c4 = same ? m4 : 0;
+                                uint32_t c4h = same ? m4h : 0u;
+                                row_allsum96(c4, c4h);
+                                if (key != mrun.d) {
+                                    if (key == mprev.d) {
+                                        const MomRun t = mrun;
+                                        mrun = mprev;
+                                        mprev = t;
+                                    } else {
+                                        if (tl == 0) mom_flush(mprev, mbase, mstride);
+                                        mprev = mrun;
+                                        mrun = MomRun{};
+                                        mrun.d = key;
+                                    }
+                                }
+                                mrun.n += cn;
+                                mrun.s1 += c1;
+                                mrun.s2l += c2;
+                                mrun.s2h += c2h + (mrun.s2l < c2 ? 1u : 0u);
+                                mrun.s4l += c4;
+                                mrun.s4h += c4h + (mrun.s4l < c4 ? 1u : 0u);
+                            }
+                        }
+                    }
+                }
                 de_lane += de;
                 dm_lane += dm;
                 if (!LATLDS) __threadfence_block();
@@ -426,12 +617,16 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
             if (jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
             if (jprun) atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
         }
+        if (MOMS && tl == 0) {
+            mom_flush(mrun, mbase, mstride);
+            mom_flush(mprev, mbase, mstride);
+        }
         // dE and dM of accepted moves were summed per lane; one reduction
         // over the team's sixteen lanes per launch.  In joint mode the
         // chain itself tracks M (it is needed per attempt).
         e += team_sum(de_lane);
         const int32_t dm_team = team_sum(dm_lane);
-        if (!JOINT) m += dm_team;
+        if (!TRACKM) m += dm_team;
         if (tl == 0) {
             a.D[w] = d;
             a.E[w] = e;
@@ -456,12 +651,12 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     }
 }
 
-template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, bool JOINT, int NR>
+template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, int OBS, bool LNWG, int NR>
 __global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    team_run<R, FP64, KEYING, LATLDS, LDSHIST, JOINT, NR>(a, lay, (uint64_t)blockIdx.x * kTeams,
-                                                         lds);
+    team_run<R, FP64, KEYING, LATLDS, LDSHIST, OBS, LNWG, NR>(a, lay,
+                                                             (uint64_t)blockIdx.x * kTeams, lds);
 }
 
 template <int R, bool FP64, int KEYING, int NR>
@@ -469,25 +664,46 @@ hipError_t launch_team_nr(hipStream_t st, const McsArgs &a, const TeamLayout &la
 {
     const dim3 grid((unsigned)((a.W + kTeams - 1) / kTeams)), block(kTeams * kTeam);
     const bool joint = a.joint != nullptr;
+    // per-D moments (the produce path, FP64) replace the D histogram
+    const bool moms = FP64 && a.moms != nullptr && !joint;
+    constexpr int MO = FP64 ? 2 : 0;
     void (*k)(McsArgs, TeamLayout) = nullptr;
-#define MUCA_TEAM_PICK(LL, LH)                                                  \
-    k = joint ? k_mcs_team<R, FP64, KEYING, LL, LH, true, NR>                   \
-              : k_mcs_team<R, FP64, KEYING, LL, LH, false, NR>;
-    if (lay.lat_lds && lay.lds_hist) {
-        MUCA_TEAM_PICK(true, true)
-    } else if (lay.lat_lds) {
-        MUCA_TEAM_PICK(true, false)
-    } else if (lay.lds_hist) {
-        MUCA_TEAM_PICK(false, true)
+    if (lay.lnw_global) {
+        // ln W too large for LDS (L >= 136): lattices and histogram stay in
+        // HBM as well
+        if constexpr (NR == 8) {
+            k = moms ? k_mcs_team<R, FP64, KEYING, false, false, MO, true, NR>
+                : joint ? k_mcs_team<R, FP64, KEYING, false, false, 1, true, NR>
+                        : k_mcs_team<R, FP64, KEYING, false, false, 0, true, NR>;
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else if (moms) {
+        k = lay.lat_lds ? k_mcs_team<R, FP64, KEYING, true, false, MO, false, NR>
+                        : k_mcs_team<R, FP64, KEYING, false, false, MO, false, NR>;
     } else {
-        MUCA_TEAM_PICK(false, false)
-    }
+#define MUCA_TEAM_PICK(LL, LH)                                                  \
+    k = joint ? k_mcs_team<R, FP64, KEYING, LL, LH, 1, false, NR>               \
+              : k_mcs_team<R, FP64, KEYING, LL, LH, 0, false, NR>;
+        if (lay.lat_lds && lay.lds_hist) {
+            MUCA_TEAM_PICK(true, true)
+        } else if (lay.lat_lds) {
+            MUCA_TEAM_PICK(true, false)
+        } else if (lay.lds_hist) {
+            MUCA_TEAM_PICK(false, true)
+        } else {
+            MUCA_TEAM_PICK(false, false)
+        }
 #undef MUCA_TEAM_PICK
+    }
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lay.bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, grid, block, lay.bytes, st, a, lay);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess && moms && a.nup > 0)
+        e = launch_hist_from_moments(st, a.moms, a.N, a.joint_blocks, a.hist);
+    return e;
 }
 
 template <int R, bool FP64, int KEYING>

@@ -213,9 +213,10 @@ def equilibrium_nupdates(L, factor=4):
 
 
 def protocol_instances(L, beta, jacks, walkers, mode, *, s0=50.0, growth=2.0, smax=3200.0,
-                       seed0=777):
+                       seed0=777, jack0=0):
     """Instances of one weight problem, one per jack (independent chains:
-    Philox seed seed0 + 1000 j, walker ids 0..walkers-1 in each).
+    Philox seed seed0 + 1000 j for j = jack0 .. jack0 + jacks - 1, walker ids
+    0..walkers-1 in each).
 
     mode "plain": the reference's defaults (ln W = 0, NUPDATES = 1, the
     x1.01 schedule once the histogram spans all bins, random start).
@@ -226,7 +227,7 @@ def protocol_instances(L, beta, jacks, walkers, mode, *, s0=50.0, growth=2.0, sm
     Both use the stratified start (walkers spread over D)."""
     N = L * L
     ins = []
-    for j in range(jacks):
+    for j in range(jack0, jack0 + jacks):
         q = dict(L=L, walkers=walkers, seed=seed0 + 1000 * j, beta=beta, recursion=False)
         if mode == "plain":
             q["init_nup"] = 1

@@ -1,6 +1,7 @@
 """Full-size lattices on the GPU: L = 128 (the team kernel with lattices in
 HBM instead of LDS, the lane kernel with 64 KB of ln W in LDS) and L = 256
-(65 536 spins, 65 537 D-bins: ln W fits no LDS, the generic kernel).
+(65 536 spins, 65 537 D-bins: ln W fits no LDS; the generic kernel, and
+the team kernel reading ln W from HBM one window per team).
 
 Every walker is an independent chain, so a sample of walkers pins the
 launch: the oracle replays walkers 0..7 (and the last walker) bit for bit;
@@ -31,8 +32,12 @@ def _weights(N, beta):
     (128, 96, M.KERNEL_GENERIC, M.REF32),
     (256, 64, M.KERNEL_GENERIC, M.REF32),
     (256, 64, M.KERNEL_GENERIC, M.FP64),
+    (256, 48, M.KERNEL_TEAM, M.REF32),
+    (256, 48, M.KERNEL_TEAM, M.FP64),
+    (200, 40, M.KERNEL_TEAM, M.FP64),
 ], ids=["L128-team-f32", "L128-team-f64", "L128-lane-f32", "L128-generic-f32",
-        "L256-generic-f32", "L256-generic-f64"])
+        "L256-generic-f32", "L256-generic-f64", "L256-team-f32", "L256-team-f64",
+        "L200-team-f64"])
 def test_large_lattice_walker_sample(orc, L, W, kernel, prec):
     N = L * L
     beta = 2.0

@@ -217,15 +217,20 @@ def test_produce_joint_histograms_per_block(orc, kernel):
         assert np.array_equal(gM[b * bw:(b + 1) * bw], m)
 
 
+@pytest.mark.parametrize("kernel", [M.KERNEL_GENERIC, M.KERNEL_TEAM], ids=["generic", "team"])
 @pytest.mark.parametrize("L,W,B,ordered,nup_mult", [
     (8, 96, 3, False, 7),      # random start, many D changes
+    (72, 48, 3, False, 3),     # three rounds per half-row, the last one partial
     (64, 128, 2, True, 10),    # ordered start: sum M^4 per D passes 2^64 (high words)
     (256, 64, 1, True, 0),     # |M| = 2^16 at the start: M^4 = 2^64 exactly
-], ids=["L8-random", "L64-ordered", "L256-ordered"])
-def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult):
+    (256, 32, 2, False, 0),    # random start, rough ln W read from HBM (team)
+], ids=["L8-random", "L72-random", "L64-ordered", "L256-ordered", "L256-random"])
+def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult, kernel):
     """Per-D moments accumulated by the update kernel (OBS_MOMENTS) in place
     of the joint histogram: identical to the oracle chain's exact 128-bit
-    sums, block by block, with the final M."""
+    sums, block by block, with the final M.  The generic kernel keeps one
+    walker's run per lane; the team kernel sums each round's attempts per D
+    over its sixteen lanes."""
     N = L * L
     nup = nup_mult * N + 1 if nup_mult else 2000
     rng = np.random.default_rng(L)
@@ -236,8 +241,8 @@ def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult):
     else:
         lat, E, D, Mg = oracle_init(orc, M.KEYING_PRODUCE, L, W, seed=100)
     with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B,
-                   observables=M.OBS_MOMENTS) as ctx:
-        assert ctx.info.kernel == M.KERNEL_GENERIC and ctx.info.joint_bins == 0
+                   observables=M.OBS_MOMENTS, kernel=kernel) as ctx:
+        assert ctx.info.kernel == kernel and ctx.info.joint_bins == 0
         ctx.set_lattice(lat.T)
         ctx.set_state(E, D, Mg)
         ctx.set_log_weight(lnw)

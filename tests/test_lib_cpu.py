@@ -89,8 +89,8 @@ def test_bad_arguments_raise():
 @pytest.mark.parametrize("kw,msg", [
     (dict(observables=7), "observables must be"),
     (dict(precision=M.REF32, observables=M.OBS_MOMENTS), "MUCA_FP64 mode only"),
-    (dict(precision=M.FP64, observables=M.OBS_MOMENTS, kernel=M.KERNEL_TEAM),
-     "generic kernel only"),
+    (dict(precision=M.FP64, observables=M.OBS_MOMENTS, kernel=M.KERNEL_LANE),
+     "generic and team kernels only"),
 ])
 def test_moments_mode_arguments_checked_before_any_device_call(kw, msg):
     """muca_create validates the produce-path options before it touches a
