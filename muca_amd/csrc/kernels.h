@@ -57,6 +57,7 @@ struct TeamLayout {
     uint32_t off_fde;     // (T)(-dE * beta) for dE = -8 .. 8
     uint32_t ring;        // Philox blocks per key held in a team's ring
     uint32_t off_win;     // inside a team's area: its ln W window (ln W in HBM)
+    uint32_t off_mom;     // inside a team's area: its per-D moment slots (moments)
     uint32_t off_team, off_tab, off_w, off_hist, off_lat;
     uint32_t lat_stride;  // LDS bytes per walker lattice (>= N, bank-spread)
     size_t bytes;         // dynamic LDS per workgroup
@@ -165,6 +166,11 @@ hipError_t launch_acclog_fast(hipStream_t st, uint32_t k0, uint32_t n, float *ou
 // 2^32 into each word, so the words are exact while a bin holds fewer than
 // 2^32 entries (S0 < 2^32); launch_moments_fp64 flags bins beyond that.
 constexpr int kMomWords = 8;
+// team kernel: per team a window of kMomSlots consecutive D values, each
+// slot kMomLdsWords uint64 in LDS (count, sum |M|, sum M^2, and sum M^4 as
+// three 32-bit limb sums), flushed into the words above when D leaves it
+constexpr int kMomSlots = 16;
+constexpr int kMomLdsWords = 6;
 // FP64 moments [block][4][N+1] (k_joint_moments' layout); *range_flag (zeroed
 // by the caller) is set to 1 if some bin has S0 >= 2^32
 hipError_t launch_moments_fp64(hipStream_t st, const uint64_t *moms, uint32_t N,

@@ -26,11 +26,17 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     const size_t row = a.L / 2;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     lay.row = (uint32_t)row;
-    // Philox ring: a half-row needs row/4 blocks per key and a generation pass
-    // may run up to 7 blocks ahead of them; then the team's ln W window
-    lay.ring = (uint32_t)(row / 4 + 8);
+    // Philox ring: a chunk of a half-row (at most 64 attempts) needs chunk/4
+    // blocks per key and a generation pass may run up to 7 blocks ahead of
+    // them; then the team's ln W window
+    lay.ring = (uint32_t)((row < 64 ? row : 64) / 4 + 8);
     lay.off_win = (uint32_t)al(2 * (size_t)lay.ring * 16);
     lay.team_bytes = (uint32_t)(lay.off_win + al((kWin + 2) * tsz));
+    // per-D moments (produce path, FP64): the team's window of moment slots;
+    // they replace the D histogram
+    const bool moms = fp64 && a.moms != nullptr && a.joint == nullptr;
+    lay.off_mom = lay.team_bytes;
+    if (moms) lay.team_bytes += (uint32_t)(kMomLdsWords * kMomSlots * sizeof(uint64_t));
     size_t off = 0;
     lay.off_team = 0;
     off += al((size_t)lay.team_bytes * kTeams);
@@ -45,7 +51,7 @@ bool team_layout(const McsArgs &a, int fp64, size_t lds_max, TeamLayout &lay)
     lay.lnw_global = off + wbytes > lds_max;
     if (!lay.lnw_global) off += wbytes;
     // histogram and lattice in LDS when they fit (histogram first)
-    const bool hist_fits = off + al((a.N + 1) * 4) <= lds_max;
+    const bool hist_fits = !moms && off + al((a.N + 1) * 4) <= lds_max;
     lay.lds_hist = hist_fits;
     lay.off_hist = (uint32_t)off;
     if (hist_fits) off += al((a.N + 1) * 4);

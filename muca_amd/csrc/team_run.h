@@ -7,7 +7,7 @@
 // even-L site order is column-parity stripes (muca_delta_multigpu.cu:173):
 // a half-row of L/2 same-parity sites contains no two neighbours, and every
 // neighbour of a half-row was last written by an earlier half-row.  So per
-// half-row the team
+// chunk of a half-row (up to 64 attempts) the team
 //   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
 //      the proposed spin, dE, dDel, (T)(-dE*beta) and an acceptance bitmask
 //      over a window of kWin D values around the D the row starts from,
@@ -25,7 +25,7 @@
 //   acclog(r1) < diff.  Results are bit-identical to the one-lane kernel.
 // Requires L % 8 == 0 (whole Philox blocks per half-row) and L <= 256.
 //
-// LDS (dynamic): per team a ring of Philox blocks [2 keys][L/8 + 8][4]; the
+// LDS (dynamic): per team a ring of Philox blocks [2 keys][min(L/2, 64)/4 + 8][4]; the
 // acceptance-log table; (T)(-dE*beta) for dE = -8..8; ln W with guard entries
 // on each side; the D histogram (u32, when it fits); the workgroup's lattices
 // [N][16] int8 (when they fit).
@@ -42,6 +42,7 @@ namespace team_detail {
 
 constexpr int kTeam = 16;
 constexpr int kTeams = 16;   // per workgroup of 256 lanes
+constexpr uint32_t kXcds = 8;   // accelerator dies of an MI355X, one L2 each
 // D values per acceptance mask.  Narrower windows cut the mask work per
 // attempt and make the exact replay of a round (D left the window) more
 // frequent.  Measured update-kernel times (L = 72, 4096 walkers, weight
@@ -73,41 +74,6 @@ __device__ __forceinline__ uint32_t sign_word(double x)
     return (uint32_t)(__double_as_longlong(x) >> 32);
 }
 
-// value of lane (i - K) mod 16 of this lane's 16-lane row (v_mov_b32_dpp row_ror:K)
-template <int K>
-__device__ __forceinline__ uint32_t row_ror(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int32_t)v, 0x120 + K, 0xf, 0xf, false);
-}
-
-// sum over the 16 lanes of this lane's row, in every lane of it
-__device__ __forceinline__ uint32_t row_allsum(uint32_t v)
-{
-    v += row_ror<8>(v);
-    v += row_ror<4>(v);
-    v += row_ror<2>(v);
-    v += row_ror<1>(v);
-    return v;
-}
-
-// the same for the 96-bit value lo + hi 2^64
-__device__ __forceinline__ void row_allsum96(uint64_t &lo, uint32_t &hi)
-{
-    auto step = [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        const uint64_t o = (uint64_t)row_ror<K>((uint32_t)lo) |
-                           ((uint64_t)row_ror<K>((uint32_t)(lo >> 32)) << 32);
-        const uint32_t oh = row_ror<K>(hi);
-        const uint64_t s = lo + o;
-        hi = hi + oh + (s < lo ? 1u : 0u);
-        lo = s;
-    };
-    step(std::integral_constant<int, 8>());
-    step(std::integral_constant<int, 4>());
-    step(std::integral_constant<int, 2>());
-    step(std::integral_constant<int, 1>());
-}
-
 // value of lane L of this lane's 16-lane row (v_mov_b32_dpp row_newbcast:L;
 // every lane of the row reads a valid source, so no old value is needed)
 template <int L>
@@ -119,7 +85,9 @@ __device__ __forceinline__ int32_t row_bcast(int32_t v)
 }  // namespace team_detail
 using namespace team_detail;
 
-// NR: rounds of 16 attempts per half-row, NR * 16 >= L / 2
+// NR: rounds of 16 attempts per chunk; a half-row of L / 2 attempts runs in
+// chunks of up to 16 NR (NR <= 4 keeps the registers of all rounds' attempts
+// low enough for more wavefronts per SIMD)
 // OBS: 0 the D histogram; 1 joint H(D,M) per walker block (produce2_multigpu.cu:489);
 // 2 exact per-D moments per walker block (the D histogram is then their S0).
 // LNWG: ln W read from HBM (it does not fit in LDS), one window per team.
@@ -163,6 +131,10 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
             wt[i] = v;
         }
     }
+    // MOMS: this team's moment slots, word k of slot s at msl[k * kMomSlots + s]
+    uint64_t *msl = (uint64_t *)(tbase + lay.off_mom);
+    if (MOMS)
+        for (int k = 0; k < kMomLdsWords; k++) msl[k * kMomSlots + tl] = 0;
     if (LDSHIST)
         for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = a.hist_preload;
     acclog_table_to_lds(logtab, tid, blockDim.x);
@@ -188,16 +160,34 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         int32_t d = a.D[w], e = a.E[w], m = a.M[w];
         uint64_t *jbase = nullptr;
         if (JOINT) jbase = a.joint + (w / a.block_walkers) * a.joint_stride;
-        // MOMS: the team's production attempts at one D, summed over its lanes
-        // round by round and kept for the D in use and the one before it (the
-        // same in every lane of the team), added to the block's moments when a
-        // third D comes up
+        // MOMS: the walker's production attempts are summed per D in the
+        // team's LDS slots for D = mwb .. mwb + kMomSlots - 1 (mwb the same in
+        // every lane of the team) and added to the block's moments in HBM when
+        // D leaves that window and at the end
         const uint64_t mstride = (uint64_t)N + 1;
         uint64_t *mbase = nullptr;
         if (MOMS) mbase = a.moms + (w / a.block_walkers) * (uint64_t)kMomWords * mstride;
-        MomRun mrun{}, mprev{};
-        mrun.d = -1;
-        mprev.d = -1;
+        int32_t mwb = -(1 << 30);   // no window yet
+        // lane tl moves slot tl (D = mwb + tl) to HBM and clears it
+        auto flush_slot = [&]() {
+            uint64_t v[kMomLdsWords];
+#pragma unroll
+            for (int k = 0; k < kMomLdsWords; k++) v[k] = msl[k * kMomSlots + tl];
+            if (v[0]) {
+                const uint64_t dd = (uint64_t)(mwb + (int32_t)tl);   // a visited D, in [0, N]
+                atomicAdd((unsigned long long *)&mbase[dd], (unsigned long long)v[0]);
+                atomicAdd((unsigned long long *)&mbase[mstride + dd], (unsigned long long)v[1]);
+                atomic_add_limbs(&mbase[2 * mstride + dd], &mbase[3 * mstride + dd],
+                                 &mbase[4 * mstride + dd], v[2], 0);
+                // sum M^4 = v3 + v4 2^32 + v5 2^64 (limb sums) as a 128-bit value
+                const uint64_t lo = v[3] + (v[4] << 32);
+                const uint64_t hi = v[5] + (v[4] >> 32) + (lo < v[3] ? 1u : 0u);
+                atomic_add_limbs(&mbase[5 * mstride + dd], &mbase[6 * mstride + dd],
+                                 &mbase[7 * mstride + dd], lo, hi);
+#pragma unroll
+                for (int k = 0; k < kMomLdsWords; k++) msl[k * kMomSlots + tl] = 0;
+            }
+        };
         // joint-histogram entries of this lane's attempts, counted for the bin
         // in use and the one before it and added with one atomic when a third
         // bin comes up (a walker stepping back and forth adds nothing)
@@ -211,18 +201,21 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         const bool may_wrap = a.hist_preload != 0 || (uint64_t)kTeams * a.nup >= (1ull << 32);
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
-            for (uint64_t rs = 0; rs < n; rs += Rw) {
-                const uint32_t cnt = (uint32_t)min((uint64_t)Rw, n - rs);
+            // A half-row is worked in chunks of up to 16 NR attempts (no two
+            // sites of a half-row are neighbours, so any split of it is exact)
+            for (uint64_t rs = 0; rs < n;) {
                 const uint32_t pos0 = (uint32_t)(rs % N);
                 const uint32_t h = pos0 / NH;
                 const uint32_t y = (pos0 - h * NH) / Rw;
+                const uint32_t jb = pos0 - h * NH - y * Rw;   // the chunk's place in its half-row
+                const uint32_t cnt = (uint32_t)min((uint64_t)min(Rw - jb, 16u * NR), n - rs);
                 // ---- A1: the walker's Philox stream (block b uses counter b + 1;
-                // the stream runs on across half-rows and phases) is generated
+                // the stream runs on across chunks and phases) is generated
                 // eight blocks x two keys per pass into a ring of `RB` blocks per
-                // key, so no lane idles when a half-row's block count is not a
+                // key, so no lane idles when a chunk's block count is not a
                 // multiple of eight.  Blocks [blk0, blk0 + nblk) serve this
-                // half-row; a pass may run up to 7 blocks ahead, and RB >=
-                // nblk + 8 keeps those from overwriting blocks still in use.
+                // chunk; a pass may run up to 7 blocks ahead, and RB >= nblk + 8
+                // keeps those from overwriting blocks still in use.
                 const uint32_t nblk = (cnt + 3) / 4;
                 const uint32_t blk0 = c0base + (uint32_t)(rs / 4);
                 while (gen < blk0 + nblk) {
@@ -239,8 +232,8 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     gen += 8;
                     gen_slot = gen_slot + 8 >= RB ? gen_slot + 8 - RB : gen_slot + 8;
                 }
-                // ring slot of this half-row's first block (blocks are contiguous
-                // from one half-row to the next, across the phase change too)
+                // ring slot of this chunk's first block (blocks are contiguous
+                // from one chunk to the next, across the phase change too)
                 const uint32_t slot0 = cur_slot;
                 cur_slot = cur_slot + nblk >= RB ? cur_slot + nblk - RB : cur_slot + nblk;
                 wave_sync();
@@ -307,14 +300,14 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 };
                 // Branch-free over the rounds so that the loads of all rounds
                 // can be in flight together: a lane past the end of the
-                // half-row works on a clamped (in-bounds) attempt and keeps
+                // chunk works on a clamped (in-bounds) attempt and keeps
                 // nothing of it.
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     const uint32_t j = 16 * r + tl;
                     const bool valid = j < cnt;
                     const uint32_t jc = valid ? j : cnt - 1;
-                    const uint32_t x = 2 * jc + h;
+                    const uint32_t x = 2 * (jb + jc) + h;
                     const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
                     const uint32_t xl = (x == 0) ? L - 1 : x - 1;
                     const int32_t old_s = sp(y * L + x);
@@ -475,7 +468,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     MUCA_STEPS(Fast, ab)
                     int32_t o = recover(r, off0, ab);
                     // a step started outside the window iff one before it in the
-                    // round ended there (a lane past the half-row's end repeats
+                    // round ended there (a lane past the chunk's end repeats
                     // the last step's D)
                     const bool out = (uint32_t)o >= (uint32_t)kWin && tl + 1 < min(left, 16u);
                     if (__builtin_amdgcn_ballot_w64(out)) {   // rare: replay exactly
@@ -511,7 +504,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     if (j < cnt) {
                         if ((accb[r] >> tl) & 1u) {
                             const int32_t ds = ((info[r] >> 7) & 7) - 2;
-                            int8_t &s = sp(y * L + 2 * j + h);
+                            int8_t &s = sp(y * L + 2 * (jb + j) + h);
                             s = (int8_t)(s + ds);
                             de += ((info[r] >> 2) & 31) - 8;
                             dm += ds;
@@ -554,54 +547,55 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     }
                 }
                 if (MOMS && phase) {
-                    // the round's attempts grouped by D (a round visits a few
-                    // D values), each group summed over the team's lanes and
-                    // added to the team's two-entry cache
+                    // Attempt j records the (D, M) after its step.  Steps that
+                    // are not accepted keep the state, so a round splits into
+                    // segments that start at step 0 and at every accepted step
+                    // and share one (D, M); the first lane of each segment adds
+                    // len x (1, |M|, M^2, M^4) to the slot of its D.
 #pragma unroll
                     for (int r = 0; r < NR; r++) {
                         if (16 * r >= cnt) break;
+                        const uint32_t left = min(cnt - 16 * r, 16u);
+                        const uint32_t ab = accb[r];
+                        const bool lead = tl < left && (tl == 0 || ((ab >> tl) & 1u));
+                        const uint32_t later = ab & ~upto;
+                        const uint32_t len =
+                            (later ? (uint32_t)__builtin_ctz(later) : left) - (uint32_t)tl;
                         const int32_t dd = dv[r];
-                        const int32_t mm = min(max(mv[r], -dd), dd);
-                        const uint32_t am = (uint32_t)(mm < 0 ? -mm : mm);
-                        const uint64_t m2 = (uint64_t)am * am;
-                        const uint64_t m4 = m2 * m2;
-                        const uint32_t m4h = (uint32_t)__umul64hi(m2, m2);
-                        uint32_t rem =
-                            (uint32_t)(__builtin_amdgcn_ballot_w64(16 * r + tl < cnt) >> sh) &
-                            0xffffu;
-                        // rem is the same in the sixteen lanes of a team
-                        while (__builtin_amdgcn_ballot_w64(rem != 0)) {
-                            if (rem) {
-                                const int32_t key = __shfl(dd, __builtin_ctz(rem), kTeam);
-                                const bool same = ((rem >> tl) & 1u) && dd == key;
-                                rem &= ~((uint32_t)(__builtin_amdgcn_ballot_w64(same) >> sh) &
-                                         0xffffu);
-                                const uint32_t cn = row_allsum(same ? 1u : 0u);
-                                const uint32_t c1 = row_allsum(same ? am : 0u);
-                                uint64_t c2 = same ? m2 : 0;
-                                uint32_t c2h = 0;
-                                row_allsum96(c2, c2h);
-                                uint64_t c4 = same ? m4 : 0;
-                                uint32_t c4h = same ? m4h : 0u;
-                                row_allsum96(c4, c4h);
-                                if (key != mrun.d) {
-                                    if (key == mprev.d) {
-                                        const MomRun t = mrun;
-                                        mrun = mprev;
-                                        mprev = t;
-                                    } else {
-                                        if (tl == 0) mom_flush(mprev, mbase, mstride);
-                                        mprev = mrun;
-                                        mrun = MomRun{};
-                                        mrun.d = key;
-                                    }
-                                }
-                                mrun.n += cn;
-                                mrun.s1 += c1;
-                                mrun.s2l += c2;
-                                mrun.s2h += c2h + (mrun.s2l < c2 ? 1u : 0u);
-                                mrun.s4l += c4;
-                                mrun.s4h += c4h + (mrun.s4l < c4 ? 1u : 0u);
+                        int32_t slot = dd - mwb;
+                        const bool out = lead && (uint32_t)slot >= (uint32_t)kMomSlots;
+                        const uint64_t ob = __builtin_amdgcn_ballot_w64(out);
+                        if (ob) {
+                            if ((ob >> sh) & 0xffffu) {   // the whole team: new window
+                                flush_slot();
+                                mwb = row_bcast<0>(dd) - kMomSlots / 2;
+                                slot = dd - mwb;
+                            }
+                        }
+                        if (lead) {
+                            const int32_t mm = min(max(mv[r], -dd), dd);
+                            const uint64_t am = (uint64_t)(mm < 0 ? -mm : mm);
+                            const uint64_t m2 = am * am;
+                            const uint64_t q = m2 * m2, qh = __umul64hi(m2, m2);   // M^4
+                            const uint64_t pl = q * len, ph = qh * len + __umul64hi(q, len);
+                            const uint64_t v[kMomLdsWords] = {len, len * am, len * m2,
+                                                              pl & 0xFFFFFFFFull, pl >> 32, ph};
+                            if ((uint32_t)slot < (uint32_t)kMomSlots) {
+                                // no test for zero words: a branch per word
+                                // costs more than the add
+#pragma unroll
+                                for (int k = 0; k < kMomLdsWords; k++)
+                                    atomicAdd((unsigned long long *)&msl[k * kMomSlots + slot],
+                                              (unsigned long long)v[k]);
+                            } else {
+                                // D moved more than kMomSlots / 2 within the round
+                                atomicAdd((unsigned long long *)&mbase[dd], (unsigned long long)v[0]);
+                                atomicAdd((unsigned long long *)&mbase[mstride + dd],
+                                          (unsigned long long)v[1]);
+                                atomic_add_limbs(&mbase[2 * mstride + dd], &mbase[3 * mstride + dd],
+                                                 &mbase[4 * mstride + dd], v[2], 0);
+                                atomic_add_limbs(&mbase[5 * mstride + dd], &mbase[6 * mstride + dd],
+                                                 &mbase[7 * mstride + dd], pl, ph);
                             }
                         }
                     }
@@ -610,6 +604,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 dm_lane += dm;
                 if (!LATLDS) __threadfence_block();
                 wave_sync();
+                rs += cnt;
             }
             c0base += (uint32_t)((n + 3) / 4);
         }
@@ -617,10 +612,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
             if (jrun) atomicAdd((unsigned long long *)&jbase[jbin], jrun);
             if (jprun) atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
         }
-        if (MOMS && tl == 0) {
-            mom_flush(mrun, mbase, mstride);
-            mom_flush(mprev, mbase, mstride);
-        }
+        if (MOMS) flush_slot();
         // dE and dM of accepted moves were summed per lane; one reduction
         // over the team's sixteen lanes per launch.  In joint mode the
         // chain itself tracks M (it is needed per attempt).
@@ -651,12 +643,22 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     }
 }
 
+// LNWG launches are the large lattices (L >= 136) with many walkers: they
+// are held to 128 VGPRs, four wavefronts per SIMD
 template <int R, bool FP64, int KEYING, bool LATLDS, bool LDSHIST, int OBS, bool LNWG, int NR>
-__global__ __launch_bounds__(256) void k_mcs_team(McsArgs a, TeamLayout lay)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LNWG ? 4 : 1))) void
+k_mcs_team(McsArgs a, TeamLayout lay)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    team_run<R, FP64, KEYING, LATLDS, LDSHIST, OBS, LNWG, NR>(a, lay,
-                                                             (uint64_t)blockIdx.x * kTeams, lds);
+    // Workgroups go to the eight XCDs round-robin, and eight consecutive
+    // groups of 16 walkers share each 128-byte line of the site-major
+    // lattice: workgroup b takes group (b mod 8) (G / 8) + b / 8, so that
+    // XCD x runs the consecutive groups x G / 8 .. (x + 1) G / 8 - 1 and the
+    // eight sharers of a line use one L2
+    const uint32_t G = gridDim.x;
+    uint32_t g = blockIdx.x;
+    if (G % kXcds == 0) g = (g % kXcds) * (G / kXcds) + g / kXcds;
+    team_run<R, FP64, KEYING, LATLDS, LDSHIST, OBS, LNWG, NR>(a, lay, (uint64_t)g * kTeams, lds);
 }
 
 template <int R, bool FP64, int KEYING, int NR>
@@ -671,7 +673,7 @@ hipError_t launch_team_nr(hipStream_t st, const McsArgs &a, const TeamLayout &la
     if (lay.lnw_global) {
         // ln W too large for LDS (L >= 136): lattices and histogram stay in
         // HBM as well
-        if constexpr (NR == 8) {
+        if constexpr (NR == 4) {
             k = moms ? k_mcs_team<R, FP64, KEYING, false, false, MO, true, NR>
                 : joint ? k_mcs_team<R, FP64, KEYING, false, false, 1, true, NR>
                         : k_mcs_team<R, FP64, KEYING, false, false, 0, true, NR>;
@@ -713,8 +715,7 @@ hipError_t launch_team_r(hipStream_t st, const McsArgs &a, const TeamLayout &lay
     if (rounds <= 1) return launch_team_nr<R, FP64, KEYING, 1>(st, a, lay);
     if (rounds <= 2) return launch_team_nr<R, FP64, KEYING, 2>(st, a, lay);
     if (rounds <= 3) return launch_team_nr<R, FP64, KEYING, 3>(st, a, lay);
-    if (rounds <= 4) return launch_team_nr<R, FP64, KEYING, 4>(st, a, lay);
-    return launch_team_nr<R, FP64, KEYING, 8>(st, a, lay);
+    return launch_team_nr<R, FP64, KEYING, 4>(st, a, lay);   // longer half-rows in chunks of 64
 }
 
 }  // namespace muca

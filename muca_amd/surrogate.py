@@ -180,11 +180,16 @@ def load_corpus(path=CORPUS_PATH, exclude=()):
             continue
         L = int(k.split("_")[0][1:])
         T = float(k.split("_")[1][1:])
-        lnw = d[k]
-        rng = np.random.default_rng(1000 * L + int(round(100 * T)))
-        delta = rng.uniform(DELTAS[0], DELTAS[-1], size=len(lnw))
-        runs.append(dict(L=L, beta=1.0 / T, delta=delta, lnw=lnw))
+        runs.append(curve_run(L, T, d[k]))
     return runs
+
+
+def curve_run(L, T, lnw):
+    """One converged curve as a corpus run, each D-sample with its own Delta
+    drawn uniformly from [1.90, 2.00] by a generator fixed by (L, T)."""
+    rng = np.random.default_rng(1000 * L + int(round(100 * T)))
+    delta = rng.uniform(DELTAS[0], DELTAS[-1], size=len(lnw))
+    return dict(L=L, beta=1.0 / T, delta=delta, lnw=lnw)
 
 
 def seed_log_weight(net, L, beta, delta):

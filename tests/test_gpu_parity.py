@@ -218,28 +218,33 @@ def test_produce_joint_histograms_per_block(orc, kernel):
 
 
 @pytest.mark.parametrize("kernel", [M.KERNEL_GENERIC, M.KERNEL_TEAM], ids=["generic", "team"])
-@pytest.mark.parametrize("L,W,B,ordered,nup_mult", [
-    (8, 96, 3, False, 7),      # random start, many D changes
-    (72, 48, 3, False, 3),     # three rounds per half-row, the last one partial
-    (64, 128, 2, True, 10),    # ordered start: sum M^4 per D passes 2^64 (high words)
-    (256, 64, 1, True, 0),     # |M| = 2^16 at the start: M^4 = 2^64 exactly
-    (256, 32, 2, False, 0),    # random start, rough ln W read from HBM (team)
-], ids=["L8-random", "L72-random", "L64-ordered", "L256-ordered", "L256-random"])
-def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult, kernel):
+@pytest.mark.parametrize("L,W,B,start,nup_mult", [
+    (8, 96, 3, "random", 7),    # many D changes
+    (72, 48, 3, "random", 3),   # three rounds per half-row, the last one partial
+    (64, 128, 2, "ones", 10),   # sum M^4 per D passes 2^64 (high words)
+    (256, 64, 1, "ones", 0),    # |M| = 2^16 at the start: M^4 = 2^64 exactly
+    (256, 32, 2, "random", 0),  # rough ln W read from HBM (team)
+    (64, 32, 2, "zeros", 2),    # D climbs one bin per step from 0: 16 bins per round
+], ids=["L8-random", "L72-random", "L64-ordered", "L256-ordered", "L256-random", "L64-climb"])
+def test_produce_moments_per_block(orc, L, W, B, start, nup_mult, kernel):
     """Per-D moments accumulated by the update kernel (OBS_MOMENTS) in place
     of the joint histogram: identical to the oracle chain's exact 128-bit
     sums, block by block, with the final M.  The generic kernel keeps one
-    walker's run per lane; the team kernel sums each round's attempts per D
-    over its sixteen lanes."""
+    walker's run per lane; the team kernel adds each run of attempts at one
+    (D, M) to its LDS window of D slots, or straight to HBM when D moved
+    beyond the window within 16 steps (the climb case)."""
     N = L * L
     nup = nup_mult * N + 1 if nup_mult else 2000
     rng = np.random.default_rng(L)
-    lnw = np.cumsum(rng.normal(size=N + 1)) if not ordered else np.zeros(N + 1)
-    if ordered:
-        lat = np.ones((W, N), np.int8)
-        E, D, Mg = orc.state(L, lat)
-    else:
+    if start == "random":
+        lnw = np.cumsum(rng.normal(size=N + 1))
         lat, E, D, Mg = oracle_init(orc, M.KEYING_PRODUCE, L, W, seed=100)
+    else:
+        # "zeros": every proposal on a zero spin raises D, and a steep ln W
+        # accepts it
+        lnw = np.zeros(N + 1) if start == "ones" else 40.0 * np.arange(N + 1)
+        lat = np.full((W, N), 1 if start == "ones" else 0, np.int8)
+        E, D, Mg = orc.state(L, lat)
     with M.Context(L, W, precision=M.FP64, keying=M.KEYING_PRODUCE, joint_blocks=B,
                    observables=M.OBS_MOMENTS, kernel=kernel) as ctx:
         assert ctx.info.kernel == kernel and ctx.info.joint_bins == 0
@@ -268,7 +273,7 @@ def test_produce_moments_per_block(orc, L, W, B, ordered, nup_mult, kernel):
         hi = max(hi, max(int(x) >> 64 for x in want[3]))
     # the launch's D histogram (the blocks' S0 summed on the device)
     assert np.array_equal(ghist, ohist)
-    if ordered:
+    if start == "ones":
         assert hi > 0   # the 128-bit carry path was taken
 
 

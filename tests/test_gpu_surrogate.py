@@ -31,13 +31,10 @@ def seed_lnw():
             ins.append(dict(L=L, walkers=W, seed=100 + len(ins), beta=1 / T, recursion=False))
             meta.append((L, T))
     out, gpu = S.converge(ins, stratified=True, max_gpu_s=40.0)
-    runs = []
-    for (L, T), o in zip(meta, out):
-        if o["status"]["converged"]:
-            for d in S.DELTAS:
-                runs.append(dict(L=L, beta=1 / T, delta=d, lnw=o["lnw"]))
-    print(f"corpus: {len(runs) // 4} of {len(ins)} instances flat in {gpu:.1f} GPU s")
-    assert len(runs) >= 4 * 9
+    runs = [S.curve_run(L, T, o["lnw"]) for (L, T), o in zip(meta, out)
+            if o["status"]["converged"]]
+    print(f"corpus: {len(runs)} of {len(ins)} instances flat in {gpu:.1f} GPU s")
+    assert len(runs) >= 9
     X, t, w = S.corpus(runs)
     net, info = S.train(X, t, w, steps=6000)
     print(f"training: {info['samples']} samples, loss {info['final_loss']:.3g}, "
@@ -66,9 +63,10 @@ def test_seeded_run_beats_plain_and_schedule(seed_lnw):
               f"D_KL {st['dkl']:.2e}")
     assert seeded["status"]["converged"]
     assert plain["status"]["converged"]
-    assert seeded["gpu_s"] * 1.5 < plain["gpu_s"]
+    # the iteration counts are fixed by the (bit-exact) chains; measured 3.1x
+    assert seeded["gpu_s"] * 2.5 < plain["gpu_s"]
     # the network, not the schedule, makes the difference
-    assert not sched["status"]["converged"] or seeded["gpu_s"] * 1.5 < sched["gpu_s"]
+    assert not sched["status"]["converged"] or seeded["gpu_s"] * 2.5 < sched["gpu_s"]
     d = seeded["lnw"] - plain["lnw"]
     print(f"ln W seeded - plain: max {np.max(np.abs(d)):.3f} rms {np.sqrt(np.mean(d * d)):.3f}")
     assert np.sqrt(np.mean(d * d)) < 0.5
