@@ -509,39 +509,56 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                             de += ((info[r] >> 2) & 31) - 8;
                             dm += ds;
                         }
-                        if (phase && !MOMS) {
+                        if (phase && JOINT) {
+                            const int64_t dd = dv[r];
+                            const int64_t mm = min(max((int64_t)mv[r], -dd), dd);
+                            const int64_t jbn = (mm + dd * dd + 2 * dd) / 2;
+                            if (jbn != jbin) {
+                                if (jbn == jpbin) {
+                                    const uint64_t t = jrun;
+                                    jrun = jprun;
+                                    jprun = t;
+                                    jpbin = jbin;
+                                } else {
+                                    if (jprun)
+                                        atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
+                                    jprun = jrun;
+                                    jpbin = jbin;
+                                    jrun = 0;
+                                }
+                                jbin = jbn;
+                            }
+                            jrun++;
+                        }
+                    }
+                }
+                if (phase && !MOMS) {
+                    // The D histogram: a round's steps split into runs at one D,
+                    // starting at step 0 and at every accepted step that
+                    // changed D; the first lane of a run adds its length (one
+                    // atomic per run instead of sixteen on the same bin)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        if (16 * r >= cnt) break;
+                        const uint32_t left = min(cnt - 16 * r, 16u);
+                        const bool moved = ((accb[r] >> tl) & 1u) && (mask[r] & 0x30000u);
+                        const uint32_t starts =
+                            ((uint32_t)(__builtin_amdgcn_ballot_w64(moved) >> sh) & 0xffffu) | 1u;
+                        const uint32_t later = starts & ~upto;
+                        const uint32_t len =
+                            (later ? (uint32_t)__builtin_ctz(later) : left) - (uint32_t)tl;
+                        if (tl < left && ((starts >> tl) & 1u)) {
+                            const int32_t dd = dv[r];
                             if (LDSHIST) {
                                 if (may_wrap) {
                                     // a u32 bin that wraps carries 2^32 into HBM
-                                    if (atomicAdd(&hist[dv[r]], 1u) == 0xFFFFFFFFu)
-                                        atomicAdd((unsigned long long *)&a.hist[dv[r]],
-                                                  1ull << 32);
+                                    if (atomicAdd(&hist[dd], len) > 0xFFFFFFFFu - len)
+                                        atomicAdd((unsigned long long *)&a.hist[dd], 1ull << 32);
                                 } else {
-                                    atomicAdd(&hist[dv[r]], 1u);
+                                    atomicAdd(&hist[dd], len);
                                 }
-                            }
-                            else
-                                atomicAdd((unsigned long long *)&a.hist[dv[r]], 1ull);
-                            if (JOINT) {
-                                const int64_t dd = dv[r];
-                                const int64_t mm = min(max((int64_t)mv[r], -dd), dd);
-                                const int64_t jb = (mm + dd * dd + 2 * dd) / 2;
-                                if (jb != jbin) {
-                                    if (jb == jpbin) {
-                                        const uint64_t t = jrun;
-                                        jrun = jprun;
-                                        jprun = t;
-                                        jpbin = jbin;
-                                    } else {
-                                        if (jprun)
-                                            atomicAdd((unsigned long long *)&jbase[jpbin], jprun);
-                                        jprun = jrun;
-                                        jpbin = jbin;
-                                        jrun = 0;
-                                    }
-                                    jbin = jb;
-                                }
-                                jrun++;
+                            } else {
+                                atomicAdd((unsigned long long *)&a.hist[dd], (unsigned long long)len);
                             }
                         }
                     }
