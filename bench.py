@@ -279,6 +279,9 @@ def main():
     ap.add_argument("--blocks", type=int, default=10, help="c5: jackknife blocks")
     ap.add_argument("--weights", default="", help="c5: .npz of converged ln W (the seeded_jack* "
                     "arrays a seeded run saves); default: the surrogate seed")
+    ap.add_argument("--stratified", action="store_true",
+                    help="c5: walkers start spread over D (stratified start) instead of the "
+                    "reference's uniform random spins")
     ap.add_argument("--observables", choices=["moments", "joint"], default="moments",
                     help="c5: per-D moments (default) or the reference's joint H(D,M)")
     ap.add_argument("--mode", choices=["all", "both", "seeded", "plain", "nonet"],
@@ -629,8 +632,8 @@ def c5_workload(M, args):
             raise SystemExit(f"{args.weights}: no seeded_jack* arrays of N+1 = {N + 1} values")
         lnw, lnw_err = S.jackknife_lnw(lnw_jacks) if len(lnw_jacks) > 1 else (lnw_jacks[0], None)
         lnw = lnw - lnw[0]
-        weights_note = (f"converged ln W, jackknife mean of {len(lnw_jacks)} jacks "
-                        f"({os.path.basename(args.weights)})")
+        weights_note = (f"ln W from {os.path.basename(args.weights)}: mean of "
+                        f"{len(lnw_jacks)} seeded jacks")
     else:
         X, t, w = S.corpus(S.load_corpus())
         net, _ = S.train(X, t, w)
@@ -644,7 +647,7 @@ def c5_workload(M, args):
     obs_mode = M.OBS_JOINT if joint else M.OBS_MOMENTS
     with M.Context(L, W, keying=M.KEYING_PRODUCE, precision=M.FP64, joint_blocks=B,
                    observables=obs_mode, kernel=args.kernel) as ctx:
-        ctx.init_lattice(mm=0, gpu=0, seed=seed)
+        ctx.init_lattice(mm=0, gpu=0, seed=seed, stratified=args.stratified)
         ctx.compute_state()
         ctx.set_log_weight(lnw)
         # untimed warm-up: the first launch of a kernel in a process carries
@@ -701,7 +704,8 @@ def c5_workload(M, args):
         "dtype": "int8 spins, f64 acceptance, " +
                  ("u64 joint histograms" if joint else "exact per-D moments (u64/u128)") +
                  ", f64 reweighting",
-        "data": "synthetic (random initial spins from the reference's Philox keying)",
+        "data": "synthetic (" + ("stratified start over D" if args.stratified else
+                                 "random initial spins from the reference's Philox keying") + ")",
         "config": {"workload": f"large production, L={L}, {W} walkers, T=0.5, "
                                f"{B} jackknife blocks", "L": L, "walkers": W,
                    "therm": f"{args.therm_mult} N", "nupdates": f"{args.nup_mult} N",

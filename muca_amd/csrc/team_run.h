@@ -356,13 +356,20 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 // One chain step.  CHK = false: no memory access and no branch.
                 // CHK = true: a step that starts outside the window is decided
                 // exactly from the attempt's record in LDS.
-                auto step = [&](auto lc, auto chk, int r, uint32_t &ab) {
+                // MODE 0: no memory access and no branch.  MODE 1: a step that
+                // starts outside the window is decided exactly from the
+                // attempt's record.  MODE 2: as 0 from step kres on, steps
+                // before kres left alone.
+                uint32_t kres = 0;
+                auto step = [&](auto lc, auto mode, int r, uint32_t &ab) {
                     constexpr int l = decltype(lc)::value;
-                    constexpr bool CHK = decltype(chk)::value;
+                    constexpr int MODE = decltype(mode)::value;
+                    constexpr bool CHK = MODE == 1;
                     const uint32_t mj = (uint32_t)row_bcast<l>((int32_t)mask[r]);
                     const int32_t ddel = __builtin_amdgcn_sbfe((int32_t)mj, 16, 2);
                     // v_bfe_u32 reads the offset modulo 32
                     uint32_t acc = __builtin_amdgcn_ubfe(mj, (uint32_t)off, 1);
+                    if (MODE == 2) acc = (uint32_t)l >= kres ? acc : 0u;
                     if (CHK) {
                         // a step outside the window: the attempt's dE from its
                         // info word, its r1 from the ring, the screened log
@@ -394,8 +401,22 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     off += (int32_t)acc * ddel;
                     ab |= acc << l;
                 };
-                using Fast = std::false_type;
-                using Checked = std::true_type;
+                using Fast = std::integral_constant<int, 0>;
+                using Checked = std::integral_constant<int, 1>;
+                using Resume = std::integral_constant<int, 2>;
+                // round r's masks recomputed around dlo by all lanes of the team
+                auto remask = [&](uint32_t &mk_r, int32_t inf_r, int r) {
+                    const T *wc = window();
+                    T wf[kWin];
+#pragma unroll
+                    for (int k = 0; k < kWin; k++) wf[k] = wc[k];
+                    uint32_t w1 = s4 + min(16 * (uint32_t)r + tl, cnt - 1);
+                    w1 = w1 >= ringw ? w1 - ringw : w1;
+                    const int32_t dd = __builtin_amdgcn_sbfe((int32_t)mk_r, 16, 2);
+                    const uint32_t mk =
+                        accept_bits(wf, wc, rng[w1], fdet[((inf_r >> 2) & 31) - 8], dd);
+                    mk_r = mk | (mk_r & 0x30000u);
+                };
 #define MUCA_STEPS(K, AB)                                                                    \
     if (left >= 16) {                                                                        \
         step(std::integral_constant<int, 0>(), K(), r, AB);                                  \
@@ -451,16 +472,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         if (shift) {
                             dlo += off - kWin / 2;
                             off = kWin / 2;
-                            const T *wc = window();
-                            T wf[kWin];
-#pragma unroll
-                            for (int k = 0; k < kWin; k++) wf[k] = wc[k];
-                            uint32_t w1 = s4 + min(16 * r + tl, cnt - 1);
-                            w1 = w1 >= ringw ? w1 - ringw : w1;
-                            const int32_t dd = __builtin_amdgcn_sbfe((int32_t)mask[r], 16, 2);
-                            const uint32_t mk = accept_bits(wf, wc, rng[w1],
-                                                            fdet[((info[r] >> 2) & 31) - 8], dd);
-                            mask[r] = mk | (mask[r] & 0x30000u);
+                            remask(mask[r], info[r], r);
                         }
                     }
                     const int32_t off0 = off;
@@ -471,13 +483,33 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     // round ended there (a lane past the chunk's end repeats
                     // the last step's D)
                     const bool out = (uint32_t)o >= (uint32_t)kWin && tl + 1 < min(left, 16u);
-                    if (__builtin_amdgcn_ballot_w64(out)) {   // rare: replay exactly
-                        const uint64_t mine = __builtin_amdgcn_ballot_w64(out) >> sh;
-                        if (mine & 0xffffu) {
-                            off = off0;
-                            ab = 0;
-                            MUCA_STEPS(Checked, ab)
-                            o = recover(r, off0, ab);
+                    if (__builtin_amdgcn_ballot_w64(out)) {
+                        const uint32_t mine =
+                            (uint32_t)(__builtin_amdgcn_ballot_w64(out) >> sh) & 0xffffu;
+                        if (mine) {
+                            // D left the window after step kres - 1: steps
+                            // 0 .. kres - 1 stand; the masks are recomputed
+                            // around D at the start of step kres and the chain
+                            // resumes there
+                            kres = (uint32_t)__builtin_ctz(mine) + 1u;
+                            const int32_t moved = __shfl(o, (int)kres - 1, kTeam) - kWin / 2;
+                            dlo += moved;
+                            remask(mask[r], info[r], r);
+                            const int32_t offr0 = off0 - moved;   // the round's start, new window
+                            off = kWin / 2;
+                            uint32_t abr = 0;
+                            MUCA_STEPS(Resume, abr)
+                            ab = (ab & ((1u << kres) - 1u)) | abr;
+                            o = recover(r, offr0, ab);
+                            // rare: D left the new window too -- replay exactly
+                            const bool out2 = (uint32_t)o >= (uint32_t)kWin &&
+                                              tl + 1 < min(left, 16u) && (uint32_t)tl >= kres;
+                            if ((uint32_t)(__builtin_amdgcn_ballot_w64(out2) >> sh) & 0xffffu) {
+                                off = offr0;
+                                ab = 0;
+                                MUCA_STEPS(Checked, ab)
+                                o = recover(r, offr0, ab);
+                            }
                         }
                     }
                     accb[r] = ab;
