@@ -10,14 +10,17 @@
 // chunk of a half-row (up to 64 attempts) the team
 //   A. computes in parallel, attempt j on lane j % 16: the Philox blocks,
 //      the proposed spin, dE, dDel, (T)(-dE*beta) and an acceptance bitmask
-//      over a window of kWin D values around the D the row starts from,
+//      over a window of 8 (weight path) or 6 (produce path) D values around
+//      the D the chunk starts from,
 //      screened with the hardware log (the exact log only where the screen
 //      is inconclusive);
 //   B. runs only the D chain serially: attempt j's mask is broadcast from
 //      its lane with a DPP row broadcast and the accept bit is read at the
 //      current D -- no memory access and no branch on the chain.  A round of
-//      16 steps in which D left the window (more than kWin/2 net moves from
-//      the row's start) is replayed with an exact per-step slow path;
+//      16 steps that starts outside the window gets its masks recomputed
+//      around the current D; one in which D leaves the window resumes, with
+//      masks recomputed, from the step that started outside (and, if D
+//      leaves that window too, is replayed with an exact per-step path);
 //   C. writes the accepted spins, sums dE and dM and fills the histogram.
 // Every acceptance is computed with the reference's operations and rounding:
 //   diff = ((T)(-dE*beta) + lnW[D+dDel]) - lnW[D];  accept iff diff >= 0 or
@@ -43,13 +46,16 @@ namespace team_detail {
 constexpr int kTeam = 16;
 constexpr int kTeams = 16;   // per workgroup of 256 lanes
 constexpr uint32_t kXcds = 8;   // accelerator dies of an MI355X, one L2 each
-// D values per acceptance mask.  Narrower windows cut the mask work per
-// attempt and make the exact replay of a round (D left the window) more
-// frequent.  Measured update-kernel times (L = 72, 4096 walkers, weight
-// iterations from ln W = 0): 16 -> 0.590 ms, 12 -> 0.572, 8 -> 0.540,
-// 6 -> 0.540; on flat L = 64 weights at T = 0.55: 6 -> 44.4 attempts/ns,
-// 8 -> 51.9, 12 -> 54.3.
+// D values per acceptance mask (the most any launch uses).  Narrower
+// windows cut the mask work per attempt and make rounds in which D leaves
+// the window more frequent.  The weight path (D moves fast in the early
+// iterations from ln W = 0) uses 8: L = 72, 4096 walkers, 40.7 attempts/ns
+// per weight-loop step against 39.8 with 6.  The produce path (D drifts
+// slowly under near-flat weights) uses 6: L = 256, 16 384 walkers, 50.3
+// attempts/ns against 41.8 with 8.
 constexpr int kWin = 8;
+template <int KEYING>
+constexpr int kWinFor = KEYING == 1 ? 6 : 8;
 constexpr int kGuard = 24;   // ln W guard entries on each side (> kWin / 2 + 2)
 
 __device__ __forceinline__ void wave_sync()
@@ -98,6 +104,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                                          unsigned char *lds)
 {
     typedef typename std::conditional<FP64, double, float>::type T;
+    constexpr int KW = kWinFor<KEYING>;   // this launch's mask width (<= kWin)
     constexpr bool JOINT = OBS == 1, MOMS = OBS == 2, TRACKM = OBS >= 1;
     __syncthreads();   // LDS may still be read by a previous call's flush
     T *wt = (T *)(lds + lay.off_w) + kGuard;   // valid for [-kGuard, N + kGuard]
@@ -109,7 +116,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
     unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
     uint32_t *rng = (uint32_t *)tbase;                          // [2][ring][4]
-    T *winb = (T *)(tbase + lay.off_win);                       // LNWG: lnW[dlo - 1 .. dlo + kWin]
+    T *winb = (T *)(tbase + lay.off_win);                       // LNWG: lnW[dlo - 1 .. dlo + KW]
     // ln W at any D: from LDS, or from HBM (clamped into [0, N]; only
     // positions a walker cannot be at read a clamped value)
     auto lnw_at = [&](int32_t i) -> T {
@@ -238,27 +245,27 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 cur_slot = cur_slot + nblk >= RB ? cur_slot + nblk - RB : cur_slot + nblk;
                 wave_sync();
                 // ---- A2: attempt j = 16 r + tl on this lane, everything in registers
-                int32_t dlo = d - kWin / 2;           // mask bit k <-> D = dlo + k
-                // lnW[dlo + k] at wcen[k], k = -1 .. kWin (LNWG: staged in the
+                int32_t dlo = d - KW / 2;           // mask bit k <-> D = dlo + k
+                // lnW[dlo + k] at wcen[k], k = -1 .. KW (LNWG: staged in the
                 // team's window from HBM)
                 auto window = [&]() -> const T * {
                     if (LNWG) {
-                        if (tl < kWin + 2) winb[tl] = lnw_at(dlo - 1 + tl);
+                        if (tl < KW + 2) winb[tl] = lnw_at(dlo - 1 + tl);
                         wave_sync();
                         return winb + 1;
                     }
                     return wt + dlo;
                 };
                 const T *wcen = window();
-                T wfrom[kWin];                        // lnW[dlo .. dlo + kWin - 1]
+                T wfrom[KW];                        // lnW[dlo .. dlo + KW - 1]
 #pragma unroll
-                for (int k = 0; k < kWin; k++) wfrom[k] = wcen[k];
+                for (int k = 0; k < KW; k++) wfrom[k] = wcen[k];
                 const int32_t dlo_a = dlo;            // the window of every mask made here
                 const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
                 const uint32_t s4 = slot0 * 4, ringw = RB * 4;   // ring word of attempt j: s4 + j (mod ringw)
                 uint32_t mask[NR];
                 int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
-                // Bits k = 0..kWin-1: accept at D = dlo + k, i.e. acclog(r1) <
+                // Bits k = 0..KW-1: accept at D = dlo + k, i.e. acclog(r1) <
                 // ((fde + lnW[D + dDel]) - lnW[D]), screened with the hardware
                 // log and its proven band: la + band < diff accepts, la - band
                 // >= diff rejects.  Bit k is the sign of e = thi - diff (e is
@@ -268,9 +275,9 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 auto accept_bits = [&](const T *wf, const T *wc, uint32_t r1, T fde,
                                        int32_t ddel) -> uint32_t {
                     const T *wto = wc + ddel;
-                    T diff[kWin];
+                    T diff[KW];
 #pragma unroll
-                    for (int k = 0; k < kWin; k++) {
+                    for (int k = 0; k < KW; k++) {
                         diff[k] = fde + wto[k];
                         diff[k] = diff[k] - wf[k];
                     }
@@ -280,7 +287,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     uint32_t mk = 0;
                     T emin = (T)1e30f;
 #pragma unroll
-                    for (int k = kWin - 1; k >= 0; k--) {
+                    for (int k = KW - 1; k >= 0; k--) {
                         const T e = thi - diff[k];
                         mk = __builtin_amdgcn_alignbit(mk, sign_word(e), 31);
                         emin = fmin(emin, fabs(e));
@@ -292,7 +299,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                                 (T)(float)acclog_u01_double_tab(r1, logtab, logtab + 128);
                             mk = 0;
 #pragma unroll
-                            for (int k = kWin - 1; k >= 0; k--)
+                            for (int k = KW - 1; k >= 0; k--)
                                 mk = mk + mk + (tlog < diff[k] ? 1u : 0u);
                         }
                     }
@@ -322,7 +329,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t dE = -ds * nb;
                     const int32_t ddel = new_s * new_s - old_s * old_s;
                     const uint32_t mk = accept_bits(wfrom, wcen, r1, fdet[dE], ddel);
-                    // bits 0..kWin-1: accept at D = dlo + k; bits 16..17: dDel
+                    // bits 0..KW-1: accept at D = dlo + k; bits 16..17: dDel
                     mask[r] = mk | (valid ? ((uint32_t)ddel & 3u) << 16 : 0u);
                     if (JOINT)
                         info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
@@ -335,7 +342,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 // window position); per step one DPP broadcast of the attempt's
                 // mask word, the accept bit at the current position, its dDel,
                 // and the move.  A position outside the window reads a zero bit
-                // (bits kWin..15 and 24..31 of the word are zero), so the chain
+                // (bits KW..15 and 24..31 of the word are zero), so the chain
                 // then stays put; the D recovered after the round shows such an
                 // excursion, and the round is replayed exactly.
                 uint32_t accb[NR];
@@ -374,7 +381,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                         // a step outside the window: the attempt's dE from its
                         // info word, its r1 from the ring, the screened log
                         // (exact only inside the band)
-                        const bool oob = (uint32_t)off >= (uint32_t)kWin;
+                        const bool oob = (uint32_t)off >= (uint32_t)KW;
                         if (__builtin_amdgcn_ballot_w64(oob)) {
                             if (oob) {
                                 const int32_t ij = row_bcast<l>(info[r]);
@@ -407,9 +414,9 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 // round r's masks recomputed around dlo by all lanes of the team
                 auto remask = [&](uint32_t &mk_r, int32_t inf_r, int r) {
                     const T *wc = window();
-                    T wf[kWin];
+                    T wf[KW];
 #pragma unroll
-                    for (int k = 0; k < kWin; k++) wf[k] = wc[k];
+                    for (int k = 0; k < KW; k++) wf[k] = wc[k];
                     uint32_t w1 = s4 + min(16 * (uint32_t)r + tl, cnt - 1);
                     w1 = w1 >= ringw ? w1 - ringw : w1;
                     const int32_t dd = __builtin_amdgcn_sbfe((int32_t)mk_r, 16, 2);
@@ -467,11 +474,11 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     // masks stay relative to dlo_a)
                     dlo = dlo_a;
                     off = d - dlo_a;
-                    const bool shift = (uint32_t)off >= (uint32_t)kWin;
+                    const bool shift = (uint32_t)off >= (uint32_t)KW;
                     if (__builtin_amdgcn_ballot_w64(shift)) {
                         if (shift) {
-                            dlo += off - kWin / 2;
-                            off = kWin / 2;
+                            dlo += off - KW / 2;
+                            off = KW / 2;
                             remask(mask[r], info[r], r);
                         }
                     }
@@ -482,7 +489,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     // a step started outside the window iff one before it in the
                     // round ended there (a lane past the chunk's end repeats
                     // the last step's D)
-                    const bool out = (uint32_t)o >= (uint32_t)kWin && tl + 1 < min(left, 16u);
+                    const bool out = (uint32_t)o >= (uint32_t)KW && tl + 1 < min(left, 16u);
                     if (__builtin_amdgcn_ballot_w64(out)) {
                         const uint32_t mine =
                             (uint32_t)(__builtin_amdgcn_ballot_w64(out) >> sh) & 0xffffu;
@@ -492,17 +499,17 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                             // around D at the start of step kres and the chain
                             // resumes there
                             kres = (uint32_t)__builtin_ctz(mine) + 1u;
-                            const int32_t moved = __shfl(o, (int)kres - 1, kTeam) - kWin / 2;
+                            const int32_t moved = __shfl(o, (int)kres - 1, kTeam) - KW / 2;
                             dlo += moved;
                             remask(mask[r], info[r], r);
                             const int32_t offr0 = off0 - moved;   // the round's start, new window
-                            off = kWin / 2;
+                            off = KW / 2;
                             uint32_t abr = 0;
                             MUCA_STEPS(Resume, abr)
                             ab = (ab & ((1u << kres) - 1u)) | abr;
                             o = recover(r, offr0, ab);
                             // rare: D left the new window too -- replay exactly
-                            const bool out2 = (uint32_t)o >= (uint32_t)kWin &&
+                            const bool out2 = (uint32_t)o >= (uint32_t)KW &&
                                               tl + 1 < min(left, 16u) && (uint32_t)tl >= kres;
                             if ((uint32_t)(__builtin_amdgcn_ballot_w64(out2) >> sh) & 0xffffu) {
                                 off = offr0;
