@@ -167,6 +167,8 @@ def mcs(L, lat, E, D, M, *, keying, fp64, lnw32=None, lnw64=None, iter=0,
     accumulated into the arrays passed in."""
     N = L * L
     assert lat.dtype == np.int8 and lat.flags.c_contiguous and lat.shape[1] == N
+    if nthreads > 1 and (joint is not None or moments is not None):
+        raise ValueError("joint histograms and moments are accumulated with nthreads=1 only")
     W = lat.shape[0]
     if hist is None:
         hist = np.zeros(N + 1, dtype=np.uint64)
