@@ -268,8 +268,9 @@ int muca_batch_get_lattice(muca_batch *batch, uint32_t instance, int8_t *site_ma
 int muca_batch_get_state(muca_batch *batch, uint32_t instance, int32_t *E, int32_t *D,
                          int32_t *M);
 /* `iterations` weight iterations of every unconverged instance; use_graph
- * replays one captured hipGraph per iteration; gpu_ms (nullable) receives
- * the elapsed device time */
+ * replays a captured hipGraph of up to 8 iterations (the remainder of
+ * `iterations` runs eagerly; results are the same either way); gpu_ms
+ * (nullable) receives the elapsed device time */
 int muca_batch_iterate(muca_batch *batch, uint32_t iterations, int32_t use_graph,
                        float *gpu_ms);
 /* the same iterations launched one by one with HIP events around each launch

@@ -274,8 +274,8 @@ struct muca_batch {
     std::vector<muca::BatchTile> tiles;
     muca::BatchArgs a{};
     muca::BatchTile *d_tiles = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr, graph8 = nullptr;   // one iteration, kGraphIters iterations
+    hipGraphExec_t exec = nullptr, exec8 = nullptr;
     uint64_t lat_bytes = 0, walkers = 0, bins = 0;
     bool team_ok = false, lane_ok = false;
     size_t lds_device = 0;
@@ -285,12 +285,46 @@ struct muca_batch {
 
 namespace {
 
+// iterations replayed by one launch of the larger graph: consecutive
+// iterations in one graph start back to back, while separate graph launches
+// leave a gap of several microseconds between them
+constexpr uint32_t kGraphIters = 8;
+
+void drop_graphs(muca_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec8) hipGraphExecDestroy(b->exec8);
+    if (b->graph8) hipGraphDestroy(b->graph8);
+    b->exec = b->exec8 = nullptr;
+    b->graph = b->graph8 = nullptr;
+}
+
+hipError_t launch_iteration(muca_batch *b);
+
+// `iters` iterations captured from the batch's stream into one graph
+hipError_t capture_iterations(muca_batch *b, uint32_t iters, hipGraph_t *g, hipGraphExec_t *x)
+{
+    hipError_t e = hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    for (uint32_t k = 0; k < iters && e == hipSuccess; k++) e = launch_iteration(b);
+    hipGraph_t gr = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->st, &gr);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(x, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (gr) hipGraphDestroy(gr);
+        return e;
+    }
+    *g = gr;
+    return hipSuccess;
+}
+
 void batch_free(muca_batch *b)
 {
     if (!b) return;
     hipSetDevice(b->device);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    drop_graphs(b);
     void *bufs[] = {b->a.inst, b->d_tiles, b->a.queue, b->a.lat, b->a.E, b->a.D, b->a.M,
                     b->a.lnw32, b->a.lnw64, b->a.hist, b->a.hist_last, b->a.bf, b->a.p_acc};
     for (void *p : bufs)
@@ -322,10 +356,7 @@ int batch_init(muca_batch *b, uint32_t mm, int stratified);
 // Tile list, grid and LDS size for one tile kind; drops any captured graph.
 hipError_t configure(muca_batch *b, int kind, uint32_t block)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    drop_graphs(b);
     b->kind = kind;
     size_t need = 0;   // LDS of the largest instance's tile
     for (const auto &I : b->inst)
@@ -649,25 +680,24 @@ int muca_batch_iterate(muca_batch *b, uint32_t iterations, int32_t use_graph, fl
 {
     if (!b) return bfail(MUCA_E_ARG, "NULL batch");
     BHIP(hipSetDevice(b->device));
+    // both graphs are made on the first call that uses them, so later calls
+    // of any length replay without capturing
     if (use_graph && !b->exec) {
-        BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = launch_iteration(b);
-        hipGraph_t g = nullptr;
-        hipError_t e2 = hipStreamEndCapture(b->st, &g);
-        if (e != hipSuccess || e2 != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return bfail(MUCA_E_HIP, std::string("graph capture: ") +
-                                         hipGetErrorString(e != hipSuccess ? e : e2));
+        hipError_t e = capture_iterations(b, 1, &b->graph, &b->exec);
+        if (e == hipSuccess) e = capture_iterations(b, kGraphIters, &b->graph8, &b->exec8);
+        if (e != hipSuccess) {
+            drop_graphs(b);
+            return bfail(MUCA_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         }
-        b->graph = g;
-        BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     BHIP(hipEventRecord(b->ev0, b->st));
-    for (uint32_t k = 0; k < iterations; k++) {
-        if (use_graph)
-            BHIP(hipGraphLaunch(b->exec, b->st));
-        else
-            BHIP(launch_iteration(b));
+    uint32_t k = 0;
+    if (use_graph) {
+        for (; k + kGraphIters <= iterations; k += kGraphIters)
+            BHIP(hipGraphLaunch(b->exec8, b->st));
+        for (; k < iterations; k++) BHIP(hipGraphLaunch(b->exec, b->st));
+    } else {
+        for (; k < iterations; k++) BHIP(launch_iteration(b));
     }
     BHIP(hipEventRecord(b->ev1, b->st));
     BHIP(hipEventSynchronize(b->ev1));

@@ -132,19 +132,25 @@ def test_batch_fp64_lane_first_iteration(orc):
 
 
 def test_batch_graph_equals_eager():
-    """Replaying the captured iteration gives the same results as launching it."""
+    """Replaying captured iterations gives the same results as launching
+    them: a graph of 3 iterations, then one of 8 replayed three times with
+    eager remainders and host reads in between (3 + 11 + 16 iterations)."""
     ins = _instances(TEAM_SET)
     out = []
     for g in (True, False):
         with M.Batch(ins, tiles=M.KERNEL_TEAM) as b:
             b.init(3)
-            b.iterate(6, use_graph=g)
-            out.append(([b.get_log_weight(i) for i in range(b.n)],
-                        [b.get_hist(i) for i in range(b.n)], b.status()))
-    for i in range(len(ins)):
-        assert np.array_equal(out[0][0][i], out[1][0][i])
-        assert np.array_equal(out[0][1][i], out[1][1][i])
-    assert out[0][2] == out[1][2]
+            snaps = []
+            for n in (3, 11, 16):
+                b.iterate(n, use_graph=g)
+                snaps.append(([b.get_log_weight(i) for i in range(b.n)],
+                              [b.get_hist(i) for i in range(b.n)], b.status()))
+            out.append(snaps)
+    for k, (sg, se) in enumerate(zip(*out)):
+        for i in range(len(ins)):
+            assert np.array_equal(sg[0][i], se[0][i]), (k, i)
+            assert np.array_equal(sg[1][i], se[1][i]), (k, i)
+        assert sg[2] == se[2], k
 
 
 def test_batch_converged_instance_stops():
