@@ -351,7 +351,7 @@ def main():
         config = {"workload": workload, "L": L, "walkers": W, "therm": "N",
                   "nupdates_init": 1, "recursive_update": int(recursion), "seed": seed,
                   "parallelism": "1 GPU; persistent update kernel over 16-walker tiles, "
-                                 "device weight update, one hipGraph per iteration"}
+                                 "device weight update, hipGraphs of eight iterations"}
     else:
         W = args.walkers or 4096
         ins, meta = batch96(W)
@@ -362,7 +362,7 @@ def main():
                   "T": [0.5, 0.6, 0.7, 0.8], "seed": "instance id", "therm": "N",
                   "recursive_update": 1,
                   "parallelism": "1 GPU; one persistent launch over all instances' tiles, "
-                                 "device weight update, one hipGraph per iteration"}
+                                 "device weight update, hipGraphs of eight iterations"}
 
     value = r["attempts"] / r["wall"] / 1e9
     avg_kernel_s = r["update_ms"] / args.steps * 1e-3
