@@ -40,10 +40,12 @@ extern "C" {
 #define MUCA_KEYING_WEIGHT 0   /* muca_delta_multigpu.cu:340-343,740-742 */
 #define MUCA_KEYING_PRODUCE 1  /* produce2_multigpu.cu:129-130,406-409 */
 
-/* update-kernel variants (MUCA_KERNEL_AUTO lets the autotuner choose) */
+/* update-kernel variants (MUCA_KERNEL_AUTO: chosen from the walkers per
+ * compute unit and the layout; muca_autotune chooses by measurement) */
 #define MUCA_KERNEL_AUTO 0
 #define MUCA_KERNEL_GENERIC 1  /* one lane per walker, reference site order from tables */
-#define MUCA_KERNEL_TEAM 2     /* sixteen lanes per walker, LDS-resident lattices (L % 8 == 0) */
+#define MUCA_KERNEL_TEAM 2     /* sixteen lanes per walker (L % 8 == 0, L <= 256; lattices and
+                                  ln W in LDS when they fit) */
 #define MUCA_KERNEL_LANE 3     /* one lane per walker, even L, table-free, screened acceptance */
 
 /* what the produce path accumulates per walker block */
