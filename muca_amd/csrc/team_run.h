@@ -264,7 +264,8 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 const uint32_t yu = (y + 1 == L) ? 0 : y + 1, yd = (y == 0) ? L - 1 : y - 1;
                 const uint32_t s4 = slot0 * 4, ringw = RB * 4;   // ring word of attempt j: s4 + j (mod ringw)
                 uint32_t mask[NR];
-                int32_t info[NR];   // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7
+                // dDel (2-bit signed) | (dE + 8) << 2 | (new - old + 2) << 7 | (new + 1) << 10
+                int32_t info[NR];
                 // Bits k = 0..KW-1: accept at D = dlo + k, i.e. acclog(r1) <
                 // ((fde + lnW[D + dDel]) - lnW[D]), screened with the hardware
                 // log and its proven band: la + band < diff accepts, la - band
@@ -328,14 +329,19 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const int32_t ds = new_s - old_s;
                     const int32_t dE = -ds * nb;
                     const int32_t ddel = new_s * new_s - old_s * old_s;
-                    const uint32_t mk = accept_bits(wfrom, wcen, r1, fdet[dE], ddel);
+                    // (T)(-dE * beta) by arithmetic (the same operations as
+                    // the table in LDS, without a dependent LDS load)
+                    const uint32_t mk =
+                        accept_bits(wfrom, wcen, r1, (T)(-(double)dE * a.beta), ddel);
                     // bits 0..KW-1: accept at D = dlo + k; bits 16..17: dDel
                     mask[r] = mk | (valid ? ((uint32_t)ddel & 3u) << 16 : 0u);
                     if (JOINT)
-                        info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7))
+                        info[r] = valid ? ((ddel & 3) | ((dE + 8) << 2) | ((ds + 2) << 7) |
+                                           ((new_s + 1) << 10))
                                         : ((8 << 2) | (2 << 7));
                     else
-                        info[r] = valid ? ((dE + 8) << 2) | ((ds + 2) << 7) : ((8 << 2) | (2 << 7));
+                        info[r] = valid ? ((dE + 8) << 2) | ((ds + 2) << 7) | ((new_s + 1) << 10)
+                                        : ((8 << 2) | (2 << 7));
                 }
                 wave_sync();
                 // ---- B: the D chain.  The chain runs on `off` = D - dlo (the
@@ -542,9 +548,10 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const uint32_t j = 16 * r + tl;
                     if (j < cnt) {
                         if ((accb[r] >> tl) & 1u) {
+                            // the new spin is known from phase A: no site of
+                            // the chunk is written between its read and here
                             const int32_t ds = ((info[r] >> 7) & 7) - 2;
-                            int8_t &s = sp(y * L + 2 * (jb + j) + h);
-                            s = (int8_t)(s + ds);
+                            sp(y * L + 2 * (jb + j) + h) = (int8_t)(((info[r] >> 10) & 3) - 1);
                             de += ((info[r] >> 2) & 31) - 8;
                             dm += ds;
                         }
