@@ -65,3 +65,40 @@ def test_large_lattice_walker_sample(orc, L, W, kernel, prec):
                 nupdates=nup, beta=beta, w0=w)
         assert np.array_equal(glat[:, w], lat[0]), w
         assert (gE[w], gD[w]) == (E[0], D[0]), w
+
+
+@pytest.mark.parametrize("rounds,keying,prec", [
+    (10, M.KEYING_WEIGHT, M.REF32),
+    (7, M.KEYING_PRODUCE, M.FP64),
+    (10, M.KEYING_PRODUCE, M.FP64),
+], ids=["r10-weight-f32", "r7-produce-f64", "r10-produce-f64"])
+def test_team_kernel_hbm_weights_rounds_and_keyings(orc, rounds, keying, prec):
+    """The team kernel at L = 256 (ln W read from HBM) with Philox4x32-10 and
+    with the produce keying (whose acceptance window is 6 D values wide
+    against 8 on the weight path): sampled walkers bit for bit."""
+    L, W = 256, 48
+    N = L * L
+    beta = 2.0
+    lnw = _weights(N, beta)
+    therm, nup = N // 2, N + 3
+    with M.Context(L, W, kernel=M.KERNEL_TEAM, precision=prec, beta=beta, rounds=rounds,
+                   keying=keying) as ctx:
+        assert ctx.info.kernel == M.KERNEL_TEAM
+        ctx.init_lattice(mm=0, gpu=0, seed=41)
+        ctx.compute_state()
+        E0, D0, M0 = ctx.get_state()
+        ctx.set_log_weight(lnw)
+        hist, _ = ctx.run(2, 555, therm, nup)
+        glat = ctx.get_lattice()
+        gE, gD, gM = ctx.get_state()
+    assert int(hist.sum()) == W * nup
+    assert np.any(gD != D0)
+    for w in [0, 1, W - 1]:
+        lat = orc.init_lattice(keying, L, 1, seed=41, w0=w, rounds=rounds)
+        E, D, Mg = orc.state(L, lat)
+        assert (E[0], D[0], Mg[0]) == (E0[w], D0[w], M0[w])
+        orc.mcs(L, lat, E, D, Mg, keying=keying, fp64=prec == M.FP64,
+                lnw32=lnw.astype(np.float32), lnw64=lnw, iter=2, seed=555, therm=therm,
+                nupdates=nup, beta=beta, rounds=rounds, w0=w)
+        assert np.array_equal(glat[:, w], lat[0]), w
+        assert (gE[w], gD[w], gM[w]) == (E[0], D[0], Mg[0]), w
