@@ -102,11 +102,19 @@ struct BatchTile {
     uint32_t inst, wbase;
 };
 
+// Tile queues of the persistent kernels: one per accelerator die (XCD) of
+// an MI355X, each with its own L2; workgroup b is dispatched to XCD b mod 8.
+// Walkers are the fastest index of a lattice, so the eight 16-walker team
+// tiles that share a 128-byte lattice line go to one XCD's queue, and that
+// line is fetched into one L2 instead of eight.
+constexpr uint32_t kBatchQueues = 8;
+
 struct BatchArgs {
     BatchInst *inst;
-    const BatchTile *tiles;
+    const BatchTile *tiles;   // queue x holds tiles[qstart[x] .. qstart[x+1])
     uint32_t ntiles, ninst;
-    uint32_t *queue;       // tile counter of the persistent kernel
+    uint32_t qstart[kBatchQueues + 1];
+    uint32_t *queue;       // kBatchQueues tile counters of the persistent kernel
     int8_t *lat;
     int32_t *E, *D, *M;
     float *lnw32;
@@ -146,7 +154,24 @@ __device__ __forceinline__ McsArgs batch_mcs_args(const BatchArgs &b, const Batc
     return a;
 }
 
-// persistent kernels: every workgroup takes tiles from *queue until none is left
+// The next tile for this workgroup: from its own XCD's queue, then from the
+// others' once that one is empty; ntiles when every queue is empty.  Each
+// queue counter passes its length at most once per workgroup, so every
+// workgroup ends after at most kBatchQueues failed takes.
+__device__ __forceinline__ uint32_t batch_take(const BatchArgs &b)
+{
+    const uint32_t x0 = blockIdx.x % kBatchQueues;
+    for (uint32_t k = 0; k < kBatchQueues; k++) {
+        const uint32_t x = (x0 + k) % kBatchQueues;
+        const uint32_t n = b.qstart[x + 1] - b.qstart[x];
+        if (n == 0) continue;
+        const uint32_t t = atomicAdd(b.queue + x, 1u);
+        if (t < n) return b.qstart[x] + t;
+    }
+    return b.ntiles;
+}
+
+// persistent kernels: every workgroup takes tiles (batch_take) until none is left
 hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
                              size_t lds_bytes);
 hipError_t launch_batch_lane(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,

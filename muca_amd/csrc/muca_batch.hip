@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
     __shared__ double sh_d[16];
     __shared__ int s_conv, s_rec;
     __shared__ double s_ds[kChunk], s_bn[kChunk], s_x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0;   // tiles of the next iteration
+    if (blockIdx.x == 0 && threadIdx.x < kBatchQueues) b.queue[threadIdx.x] = 0;   // next iteration
     BatchInst *I = b.inst + blockIdx.x;
     const uint32_t N = I->N;
     uint64_t *h = b.hist + I->w_off;
@@ -378,9 +378,25 @@ hipError_t configure(muca_batch *b, int kind, uint32_t block)
     for (uint32_t i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return b->inst[x].N > b->inst[y].N; });
-    b->tiles.clear();
+    // groups of tiles that share lattice lines (eight team tiles of 16
+    // walkers make 128 bytes; a lane tile fills whole lines by itself) are
+    // dealt to the XCD queues in turn, each queue keeping the largest-first
+    // order
+    const uint32_t group = kind == MUCA_KERNEL_TEAM ? 8 : 1;
+    std::vector<muca::BatchTile> q[muca::kBatchQueues];
+    uint32_t next = 0;
     for (uint32_t i : order)
-        for (uint32_t w = 0; w < b->inst[i].W; w += per) b->tiles.push_back({i, w});
+        for (uint32_t w = 0; w < b->inst[i].W; w += per) {
+            q[next].push_back({i, w});
+            if ((w / per) % group == group - 1 || w + per >= b->inst[i].W)
+                next = (next + 1) % muca::kBatchQueues;
+        }
+    b->tiles.clear();
+    for (uint32_t x = 0; x < muca::kBatchQueues; x++) {
+        b->a.qstart[x] = (uint32_t)b->tiles.size();
+        b->tiles.insert(b->tiles.end(), q[x].begin(), q[x].end());
+    }
+    b->a.qstart[muca::kBatchQueues] = (uint32_t)b->tiles.size();
     b->lds_bytes = need;
     const uint32_t per_cu_thr = 2048 / b->block;
     b->grid = b->cus * std::min(per_cu_lds, per_cu_thr);
@@ -500,7 +516,7 @@ int muca_batch_create(int32_t device, const muca_instance *ins, uint32_t n, int3
     A.ninst = n;
     if (e == hipSuccess) e = balloc(&A.inst, n);
     if (e == hipSuccess) e = balloc(&b->d_tiles, max_tiles);
-    if (e == hipSuccess) e = balloc(&A.queue, 1);
+    if (e == hipSuccess) e = balloc(&A.queue, muca::kBatchQueues);
     if (e == hipSuccess) e = balloc(&A.lat, lat);
     if (e == hipSuccess) e = balloc(&A.E, st);
     if (e == hipSuccess) e = balloc(&A.D, st);
@@ -600,7 +616,7 @@ int batch_init(muca_batch *b, uint32_t mm, int stratified)
     BHIP(hipStreamSynchronize(b->st));
     BHIP(hipMemcpy(A.inst, b->inst.data(), b->inst.size() * sizeof(muca::BatchInst),
                    hipMemcpyHostToDevice));
-    BHIP(hipMemset(A.queue, 0, 4));
+    BHIP(hipMemset(A.queue, 0, 4 * muca::kBatchQueues));
     BHIP(hipMemset(A.lnw32, 0, b->bins * 4));
     BHIP(hipMemset(A.lnw64, 0, b->bins * 8));
     BHIP(hipMemset(A.hist, 0, b->bins * 8));

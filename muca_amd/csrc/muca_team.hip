@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
     __shared__ uint32_t s_tile;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_tile = atomicAdd(b.queue, 1u);
+        if (threadIdx.x == 0) s_tile = batch_take(b);
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= b.ntiles) break;
