@@ -132,6 +132,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     const bool active = w < W;
 
     if (!LNWG) {
+#pragma unroll 4
         for (int i = (int)tid - kGuard; i <= (int)N + kGuard; i += blockDim.x) {
             T v = (T)0;
             if (i >= 0 && i <= (int)N) v = FP64 ? (T)a.lnw64[i] : (T)a.lnw32[i];
@@ -146,11 +147,41 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         for (uint32_t i = tid; i <= N; i += blockDim.x) hist[i] = a.hist_preload;
     acclog_table_to_lds(logtab, tid, blockDim.x);
     if (tid < 17) fdet[tid - 8] = (T)(-(double)(tid - 8) * a.beta);
+    // LATLDS: the tile's lattices move between the site-major layout in HBM
+    // and walker-major LDS rows.  For sixteen whole walkers on 16-byte
+    // aligned rows, a lane moves four consecutive sites at a time: four
+    // 16-byte global accesses (one per site: the tile's walkers), a 4x4 byte
+    // transpose per four walkers, sixteen dword LDS accesses (one per
+    // walker); a lane's four loads are all in flight before the first is
+    // used.  Otherwise one byte per lane and step.
+    const bool lat_vec = LATLDS && W % kTeams == 0 && wbase + kTeams <= W &&
+                         ((uintptr_t)a.lat & 15) == 0;   // N % 64 == 0 (L % 8 == 0)
     if (LATLDS) {
-        for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
-            const uint32_t site = e >> 4, t = e & 15;
-            latl[t * lay.lat_stride + site] =
-                (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
+        if (lat_vec) {
+            for (uint32_t q = tid; q < N / 4; q += blockDim.x) {
+                uint32_t s4[4][4];   // [site 4q+j][walkers 4c .. 4c+3]
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint4 v = *(const uint4 *)(a.lat + (uint64_t)(4 * q + j) * W + wbase);
+                    s4[j][0] = v.x;
+                    s4[j][1] = v.y;
+                    s4[j][2] = v.z;
+                    s4[j][3] = v.w;
+                }
+#pragma unroll
+                for (int t = 0; t < kTeams; t++) {
+                    const int c = t >> 2, sh = (t & 3) * 8;
+                    *(uint32_t *)(latl + t * lay.lat_stride + 4 * q) =
+                        ((s4[0][c] >> sh) & 0xffu) | (((s4[1][c] >> sh) & 0xffu) << 8) |
+                        (((s4[2][c] >> sh) & 0xffu) << 16) | (((s4[3][c] >> sh) & 0xffu) << 24);
+                }
+            }
+        } else {
+            for (uint32_t e = tid; e < N * kTeams; e += blockDim.x) {
+                const uint32_t site = e >> 4, t = e & 15;
+                latl[t * lay.lat_stride + site] =
+                    (wbase + t < W) ? a.lat[(uint64_t)site * W + wbase + t] : (int8_t)0;
+            }
         }
     }
     __syncthreads();
@@ -699,9 +730,30 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         }
     }
     if (LATLDS) {
-        for (uint32_t e2 = tid; e2 < N * kTeams; e2 += blockDim.x) {
-            const uint32_t site = e2 >> 4, t = e2 & 15;
-            if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[t * lay.lat_stride + site];
+        if (lat_vec) {
+            for (uint32_t q = tid; q < N / 4; q += blockDim.x) {
+                uint32_t r[kTeams];   // walker t's spins at sites 4q .. 4q+3
+#pragma unroll
+                for (int t = 0; t < kTeams; t++)
+                    r[t] = *(const uint32_t *)(latl + t * lay.lat_stride + 4 * q);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int sh = 8 * j;
+                    uint32_t d[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        d[c] = ((r[4 * c] >> sh) & 0xffu) | (((r[4 * c + 1] >> sh) & 0xffu) << 8) |
+                               (((r[4 * c + 2] >> sh) & 0xffu) << 16) |
+                               (((r[4 * c + 3] >> sh) & 0xffu) << 24);
+                    *(uint4 *)(a.lat + (uint64_t)(4 * q + j) * W + wbase) =
+                        make_uint4(d[0], d[1], d[2], d[3]);
+                }
+            }
+        } else {
+            for (uint32_t e2 = tid; e2 < N * kTeams; e2 += blockDim.x) {
+                const uint32_t site = e2 >> 4, t = e2 & 15;
+                if (wbase + t < W) a.lat[(uint64_t)site * W + wbase + t] = latl[t * lay.lat_stride + site];
+            }
         }
     }
 }
