@@ -234,6 +234,16 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         uint32_t c0base = 0;
         const uint32_t RB = lay.ring;
         uint32_t gen = 0, gen_slot = 0, cur_slot = 0;   // next block to make, its slot, block blk0's slot
+        // PF (lattices in HBM, ln W in LDS): the spins a chunk reads are
+        // loaded while the chunk before it runs its D chain, so the loads'
+        // latency is hidden.  pf_* hold, per round, the next chunk's site,
+        // its right, left and upper neighbours and the lower neighbour as
+        // read from memory; with pf_fwd the lower neighbour is instead this
+        // lane's own site of the chunk before (same round and lane, the row
+        // below), whose final spin fin[] phase C keeps (-9: none).
+        constexpr bool PF = !LATLDS && !LNWG;
+        int32_t pf_s[NR], pf_r[NR], pf_l[NR], pf_u[NR], pf_d[NR], fin[NR];
+        bool pf_have = false, pf_fwd = false;
         int32_t de_lane = 0, dm_lane = 0;
         // a workgroup adds at most 16 NUPDATES counts to one u32 bin
         const bool may_wrap = a.hist_preload != 0 || (uint64_t)kTeams * a.nup >= (1ull << 32);
@@ -349,9 +359,14 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     const uint32_t x = 2 * (jb + jc) + h;
                     const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
                     const uint32_t xl = (x == 0) ? L - 1 : x - 1;
-                    const int32_t old_s = sp(y * L + x);
-                    const int32_t nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) +
-                                       sp(yd * L + x);
+                    int32_t old_s, nb;
+                    if (PF && pf_have) {
+                        old_s = pf_s[r];
+                        nb = pf_r[r] + pf_l[r] + pf_u[r] + ((pf_fwd && fin[r] != -9) ? fin[r] : pf_d[r]);
+                    } else {
+                        old_s = sp(y * L + x);
+                        nb = sp(y * L + xr) + sp(y * L + xl) + sp(yu * L + x) + sp(yd * L + x);
+                    }
                     uint32_t widx = s4 + jc;
                     widx = widx >= ringw ? widx - ringw : widx;
                     const uint32_t r1 = rng[widx], r3 = rng[ringw + widx];
@@ -375,6 +390,54 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                                         : ((8 << 2) | (2 << 7));
                 }
                 wave_sync();
+                if (PF) {
+                    // The next chunk (the next phase's first after a phase's
+                    // last) and whether its spins may be read now: this chunk
+                    // writes sites jb .. jb + cnt - 1 of half-row (y, h) only.
+                    // Reads of that half-row are safe when their sites lie
+                    // outside it; the lower neighbours of the next row's
+                    // chunk at the same jb come from fin[] (lanes within this
+                    // chunk) or from memory (lanes past it: not written).
+                    uint64_t rs_n = rs + cnt, n_n = n;
+                    bool nx = rs_n < n;
+                    if (!nx && phase == 0 && a.nup > 0) {
+                        rs_n = 0;
+                        n_n = a.nup;
+                        nx = true;
+                    }
+                    const uint32_t pos_n = (uint32_t)(rs_n % N);
+                    const uint32_t h_n = pos_n / NH;
+                    const uint32_t y_n = (pos_n - h_n * NH) / Rw;
+                    const uint32_t jb_n = pos_n - h_n * NH - y_n * Rw;
+                    const uint32_t cnt_n = (uint32_t)min((uint64_t)min(Rw - jb_n, 16u * NR), n_n - rs_n);
+                    const uint32_t yu_n = (y_n + 1 == L) ? 0 : y_n + 1;
+                    const uint32_t yd_n = (y_n == 0) ? L - 1 : y_n - 1;
+                    const bool ovl = jb_n < jb + cnt && jb < jb_n + cnt_n;
+                    bool have = nx;
+                    bool fwd = false;
+                    if (y_n == y && h_n != h) have = false;          // neighbours in the written half-row
+                    if (h_n == h && (y_n == y || yu_n == y) && ovl) have = false;
+                    if (h_n == h && yd_n == y && ovl) {
+                        if (jb_n == jb) fwd = true;
+                        else have = false;
+                    }
+                    pf_have = have;
+                    pf_fwd = fwd;
+                    if (have) {
+#pragma unroll
+                        for (int r = 0; r < NR; r++) {
+                            const uint32_t jn = min(16u * r + tl, cnt_n - 1);
+                            const uint32_t x = 2 * (jb_n + jn) + h_n;
+                            const uint32_t xr = (x + 1 == L) ? 0 : x + 1;
+                            const uint32_t xl = (x == 0) ? L - 1 : x - 1;
+                            pf_s[r] = sp(y_n * L + x);
+                            pf_r[r] = sp(y_n * L + xr);
+                            pf_l[r] = sp(y_n * L + xl);
+                            pf_u[r] = sp(yu_n * L + x);
+                            pf_d[r] = sp(yd_n * L + x);
+                        }
+                    }
+                }
                 // ---- B: the D chain.  The chain runs on `off` = D - dlo (the
                 // window position); per step one DPP broadcast of the attempt's
                 // mask word, the accept bit at the current position, its dDel,
@@ -577,6 +640,13 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     const uint32_t j = 16 * r + tl;
+                    if (PF) {
+                        // the site's spin after this chunk (info: new spin + 1
+                        // at bits 10-11, new - old + 2 at bits 7-9)
+                        const int32_t ns = ((info[r] >> 10) & 3) - 1;
+                        const int32_t os = ns - (((info[r] >> 7) & 7) - 2);
+                        fin[r] = j < cnt ? (((accb[r] >> tl) & 1u) ? ns : os) : -9;
+                    }
                     if (j < cnt) {
                         if ((accb[r] >> tl) & 1u) {
                             // the new spin is known from phase A: no site of

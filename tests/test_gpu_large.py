@@ -102,3 +102,43 @@ def test_team_kernel_hbm_weights_rounds_and_keyings(orc, rounds, keying, prec):
                 nupdates=nup, beta=beta, rounds=rounds, w0=w)
         assert np.array_equal(glat[:, w], lat[0]), w
         assert (gE[w], gD[w], gM[w]) == (E[0], D[0], Mg[0]), w
+
+
+@pytest.mark.parametrize("L,prec,therm,nup", [
+    (96, M.FP64, 1, 2 * 96 * 96 + 7),
+    (160, M.REF32, 37, 160 * 160 + 5),
+    (128, M.REF32, 128 * 128 // 2 + 70, 3),
+    (136, M.REF32, 68 * 3 + 1, 136 * 68 + 64),
+], ids=["L96-f64-therm1", "L160-f32-therm37", "L128-f32-midrow", "L136-f32-rows"])
+def test_team_kernel_hbm_lattices_chunk_boundaries(orc, L, prec, therm, nup):
+    """The team kernel with lattices in HBM and ln W in LDS, where each
+    chunk's spins are loaded while the chunk before it runs: phases that end
+    inside a half-row or a chunk, a thermalisation of one attempt (the next
+    chunk reads the sites the one before wrote), and half-rows of two chunks
+    (L = 136, 160).  Sampled walkers bit for bit against the oracle chain."""
+    W = 32
+    N = L * L
+    beta = 2.0
+    lnw = _weights(N, beta)
+    with M.Context(L, W, kernel=M.KERNEL_TEAM, precision=prec, beta=beta) as ctx:
+        assert ctx.info.kernel == M.KERNEL_TEAM
+        ctx.init_lattice(mm=0, gpu=0, seed=43)
+        ctx.compute_state()
+        E0, D0, M0 = ctx.get_state()
+        ctx.set_log_weight(lnw)
+        hist, _ = ctx.run(3, 777, therm, nup)
+        glat = ctx.get_lattice()
+        gE, gD, gM = ctx.get_state()
+        ctx.compute_state()
+        rE, rD, rM = ctx.get_state()
+    assert int(hist.sum()) == W * nup
+    assert np.array_equal(gE, rE) and np.array_equal(gD, rD) and np.array_equal(gM, rM)
+    for w in [0, 5, 17, W - 1]:
+        lat = orc.init_lattice(M.KEYING_WEIGHT, L, 1, seed=43, w0=w)
+        E, D, Mg = orc.state(L, lat)
+        assert (E[0], D[0], Mg[0]) == (E0[w], D0[w], M0[w])
+        orc.mcs(L, lat, E, D, Mg, keying=M.KEYING_WEIGHT, fp64=prec == M.FP64,
+                lnw32=lnw.astype(np.float32), lnw64=lnw, iter=3, seed=777, therm=therm,
+                nupdates=nup, beta=beta, w0=w)
+        assert np.array_equal(glat[:, w], lat[0]), w
+        assert (gE[w], gD[w], gM[w]) == (E[0], D[0], Mg[0]), w
