@@ -172,12 +172,15 @@ __device__ __forceinline__ uint32_t batch_take(const BatchArgs &b)
 }
 
 // persistent kernels: every workgroup takes tiles (batch_take) until none is left
+// lat_modes: bit 0 set if some instance keeps its tile's lattices in LDS,
+// bit 1 if some reads them from HBM; nrs: the rounds per chunk (1..4) that
+// every instance has, else 0.  The kernel built for just those is launched.
 hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
-                             size_t lds_bytes);
+                             size_t lds_bytes, int lat_modes, int nrs);
 hipError_t launch_batch_lane(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
                              uint32_t block, size_t lds_bytes);
 // the kernels' dynamic-LDS limit, set once outside any graph capture
-hipError_t prepare_batch_team(int fp64, size_t lds_bytes);
+hipError_t prepare_batch_team(int fp64, size_t lds_bytes, int lat_modes, int nrs);
 hipError_t prepare_batch_lane(int fp64, size_t lds_bytes);
 // one workgroup per instance: histogram bookkeeping, convergence, ln W update
 hipError_t launch_batch_update(hipStream_t st, const BatchArgs &b);

@@ -29,9 +29,6 @@
 
 namespace muca {
 
-hipError_t prepare_batch_team(int fp64, size_t lds_bytes);
-hipError_t prepare_batch_lane(int fp64, size_t lds_bytes);
-
 namespace {
 
 constexpr int kUpd = 1024;    // threads of the update workgroup
@@ -267,6 +264,8 @@ struct muca_batch {
     int device = 0, fp64 = 0, kind = 0;   // kind: MUCA_KERNEL_TEAM | MUCA_KERNEL_LANE
     uint32_t block = 256, grid = 0, cus = 0;
     size_t lds_bytes = 0;
+    int lat_modes = 3;   // team tiles: lattices in LDS (bit 0) / in HBM (bit 1)
+    int nrs = 0;         // team tiles: rounds per chunk common to all instances, else 0
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<muca::BatchInst> inst;
@@ -344,7 +343,8 @@ hipError_t balloc(T **p, uint64_t n)
 hipError_t launch_iteration(muca_batch *b)
 {
     hipError_t e = b->kind == MUCA_KERNEL_TEAM
-                       ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes)
+                       ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes,
+                                                 b->lat_modes, b->nrs)
                        : muca::launch_batch_lane(b->st, b->fp64, b->a, b->grid, b->block,
                                                  b->lds_bytes);
     if (e != hipSuccess) return e;
@@ -398,6 +398,14 @@ hipError_t configure(muca_batch *b, int kind, uint32_t block)
     }
     b->a.qstart[muca::kBatchQueues] = (uint32_t)b->tiles.size();
     b->lds_bytes = need;
+    b->lat_modes = 0;
+    b->nrs = -1;
+    for (const auto &I : b->inst) {
+        b->lat_modes |= I.lay.lat_lds ? 1 : 2;
+        const int nr = (int)std::min<uint32_t>((I.lay.row + 15) / 16, 4);
+        b->nrs = b->nrs == -1 || b->nrs == nr ? nr : 0;
+    }
+    if (b->nrs < 0) b->nrs = 0;
     const uint32_t per_cu_thr = 2048 / b->block;
     b->grid = b->cus * std::min(per_cu_lds, per_cu_thr);
     b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->tiles.size());
@@ -405,8 +413,9 @@ hipError_t configure(muca_batch *b, int kind, uint32_t block)
     hipError_t e = hipMemcpy(b->d_tiles, b->tiles.data(),
                              b->tiles.size() * sizeof(muca::BatchTile), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    return kind == MUCA_KERNEL_TEAM ? muca::prepare_batch_team(b->fp64, b->lds_bytes)
-                                    : muca::prepare_batch_lane(b->fp64, b->lds_bytes);
+    return kind == MUCA_KERNEL_TEAM
+               ? muca::prepare_batch_team(b->fp64, b->lds_bytes, b->lat_modes, b->nrs)
+               : muca::prepare_batch_lane(b->fp64, b->lds_bytes);
 }
 
 }  // namespace
@@ -742,7 +751,8 @@ int muca_batch_iterate_profiled(muca_batch *b, uint32_t iterations, float *updat
     for (uint32_t k = 0; k < iterations; k++) {
         BHIP(hipEventRecord(ev[0], b->st));
         BHIP(b->kind == MUCA_KERNEL_TEAM
-                 ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes)
+                 ? muca::launch_batch_team(b->st, b->fp64, b->a, b->grid, b->lds_bytes,
+                                           b->lat_modes, b->nrs)
                  : muca::launch_batch_lane(b->st, b->fp64, b->a, b->grid, b->block, b->lds_bytes));
         BHIP(hipEventRecord(ev[1], b->st));
         BHIP(muca::launch_batch_update(b->st, b->a));

@@ -87,12 +87,17 @@ hipError_t launch_mcs_team(hipStream_t st, const Launch &cfg, const McsArgs &a,
 
 // ---------------------------------------------------------------------------
 // batched engine, team tiles: a persistent grid (one workgroup per compute
-// unit; each keeps ~100 KB of LDS) takes tiles of sixteen walkers from a
-// shared counter, largest lattices first, and runs one weight iteration of
-// the tile's instance.  Converged instances are skipped.  Weight keying,
+// unit; each keeps ~100 KB of LDS) takes tiles of sixteen walkers from the
+// per-XCD tile queues, largest lattices first, and runs one weight iteration
+// of the tile's instance.  Converged instances are skipped.  Weight keying,
 // Philox4x32-7, walker ids 0..W-1 within each instance, seed = instance seed.
 // ---------------------------------------------------------------------------
-template <bool FP64>
+// LM: the lattice placements this build of the kernel handles (1: LDS,
+// 2: HBM, 3: both); NRS: the rounds per chunk of every instance (0: any).
+// A batch of one placement (and one chunk size) launches the build for it
+// alone, whose registers and schedule the other variants' code does not
+// constrain.
+template <bool FP64, int LM, int NRS>
 __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
@@ -110,14 +115,28 @@ __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
         const TeamLayout lay = I->lay;
         const uint32_t nr = (lay.row + 15) / 16;
         const int v = (lay.lat_lds ? 1 : 0) | (lay.lds_hist ? 2 : 0);
-#define MUCA_BT(NR)                                                                          \
-    switch (v) {                                                                             \
-    case 0: team_run<7, FP64, 0, false, false, 0, false, NR>(a, lay, tile.wbase, lds); break; \
-    case 1: team_run<7, FP64, 0, true, false, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
-    case 2: team_run<7, FP64, 0, false, true, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
-    default: team_run<7, FP64, 0, true, true, 0, false, NR>(a, lay, tile.wbase, lds); break;  \
+#define MUCA_BT(NR)                                                                            \
+    switch (v) {                                                                               \
+    case 0:                                                                                    \
+        if constexpr ((LM & 2) != 0)                                                           \
+            team_run<7, FP64, 0, false, false, 0, false, NR>(a, lay, tile.wbase, lds);         \
+        break;                                                                                 \
+    case 1:                                                                                    \
+        if constexpr ((LM & 1) != 0)                                                           \
+            team_run<7, FP64, 0, true, false, 0, false, NR>(a, lay, tile.wbase, lds);          \
+        break;                                                                                 \
+    case 2:                                                                                    \
+        if constexpr ((LM & 2) != 0)                                                           \
+            team_run<7, FP64, 0, false, true, 0, false, NR>(a, lay, tile.wbase, lds);          \
+        break;                                                                                 \
+    default:                                                                                   \
+        if constexpr ((LM & 1) != 0)                                                           \
+            team_run<7, FP64, 0, true, true, 0, false, NR>(a, lay, tile.wbase, lds);           \
+        break;                                                                                 \
     }
-        if (nr <= 1) {
+        if constexpr (NRS != 0) {
+            MUCA_BT(NRS)
+        } else if (nr <= 1) {
             MUCA_BT(1)
         } else if (nr <= 2) {
             MUCA_BT(2)
@@ -130,17 +149,34 @@ __global__ __launch_bounds__(256) void k_batch_team(BatchArgs b)
     }
 }
 
-hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
-                             size_t lds_bytes)
+template <bool FP64>
+static void (*batch_team_kernel(int lat_modes, int nrs))(BatchArgs)
 {
-    void (*k)(BatchArgs) = fp64 ? k_batch_team<true> : k_batch_team<false>;
+    if (lat_modes == 1) {
+        switch (nrs) {
+        case 1: return k_batch_team<FP64, 1, 1>;
+        case 2: return k_batch_team<FP64, 1, 2>;
+        case 3: return k_batch_team<FP64, 1, 3>;
+        case 4: return k_batch_team<FP64, 1, 4>;
+        default: return k_batch_team<FP64, 1, 0>;
+        }
+    }
+    return lat_modes == 2 ? k_batch_team<FP64, 2, 0> : k_batch_team<FP64, 3, 0>;
+}
+
+hipError_t launch_batch_team(hipStream_t st, int fp64, const BatchArgs &b, uint32_t grid,
+                             size_t lds_bytes, int lat_modes, int nrs)
+{
+    void (*k)(BatchArgs) = fp64 ? batch_team_kernel<true>(lat_modes, nrs)
+                                : batch_team_kernel<false>(lat_modes, nrs);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kTeams * kTeam), lds_bytes, st, b);
     return hipGetLastError();
 }
 
-hipError_t prepare_batch_team(int fp64, size_t lds_bytes)
+hipError_t prepare_batch_team(int fp64, size_t lds_bytes, int lat_modes, int nrs)
 {
-    void (*k)(BatchArgs) = fp64 ? k_batch_team<true> : k_batch_team<false>;
+    void (*k)(BatchArgs) = fp64 ? batch_team_kernel<true>(lat_modes, nrs)
+                                : batch_team_kernel<false>(lat_modes, nrs);
     return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);
 }
