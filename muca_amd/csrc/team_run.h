@@ -113,7 +113,7 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
     double *logtab = (double *)(lds + lay.off_tab);
     T *fdet = (T *)(lds + lay.off_fde) + 8;    // fdet[dE] = (T)(-dE * beta), |dE| <= 8
     const int tid = threadIdx.x, team = tid >> 4, tl = tid & 15;
-    const uint32_t N = a.N, L = a.L, Rw = lay.row, NH = N / 2;
+    const uint32_t N = a.N, L = a.L, Rw = lay.row;
     unsigned char *tbase = lds + lay.off_team + (size_t)team * lay.team_bytes;
     uint32_t *rng = (uint32_t *)tbase;                          // [2][ring][4]
     T *winb = (T *)(tbase + lay.off_win);                       // LNWG: lnW[dlo - 1 .. dlo + KW]
@@ -250,12 +250,12 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
         for (int phase = 0; phase < 2; phase++) {
             const uint64_t n = phase ? a.nup : a.therm;
             // A half-row is worked in chunks of up to 16 NR attempts (no two
-            // sites of a half-row are neighbours, so any split of it is exact)
+            // sites of a half-row are neighbours, so any split of it is exact).
+            // Attempt rs of a phase is at sweep position rs mod N: half h
+            // (x parity), row y, place jb in the half-row -- advanced chunk by
+            // chunk from 0, without divisions.
+            uint32_t h = 0, y = 0, jb = 0;
             for (uint64_t rs = 0; rs < n;) {
-                const uint32_t pos0 = (uint32_t)(rs % N);
-                const uint32_t h = pos0 / NH;
-                const uint32_t y = (pos0 - h * NH) / Rw;
-                const uint32_t jb = pos0 - h * NH - y * Rw;   // the chunk's place in its half-row
                 const uint32_t cnt = (uint32_t)min((uint64_t)min(Rw - jb, 16u * NR), n - rs);
                 // ---- A1: the walker's Philox stream (block b uses counter b + 1;
                 // the stream runs on across chunks and phases) is generated
@@ -400,15 +400,20 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                     // chunk) or from memory (lanes past it: not written).
                     uint64_t rs_n = rs + cnt, n_n = n;
                     bool nx = rs_n < n;
+                    uint32_t h_n = h, y_n = y, jb_n = jb + cnt;
+                    if (jb_n == Rw) {
+                        jb_n = 0;
+                        if (++y_n == L) {
+                            y_n = 0;
+                            h_n ^= 1u;
+                        }
+                    }
                     if (!nx && phase == 0 && a.nup > 0) {
                         rs_n = 0;
                         n_n = a.nup;
                         nx = true;
+                        h_n = y_n = jb_n = 0;
                     }
-                    const uint32_t pos_n = (uint32_t)(rs_n % N);
-                    const uint32_t h_n = pos_n / NH;
-                    const uint32_t y_n = (pos_n - h_n * NH) / Rw;
-                    const uint32_t jb_n = pos_n - h_n * NH - y_n * Rw;
                     const uint32_t cnt_n = (uint32_t)min((uint64_t)min(Rw - jb_n, 16u * NR), n_n - rs_n);
                     const uint32_t yu_n = (y_n + 1 == L) ? 0 : y_n + 1;
                     const uint32_t yd_n = (y_n == 0) ? L - 1 : y_n - 1;
@@ -769,6 +774,14 @@ __device__ __forceinline__ void team_run(const McsArgs &a, const TeamLayout &lay
                 if (!LATLDS) __threadfence_block();
                 wave_sync();
                 rs += cnt;
+                jb += cnt;
+                if (jb == Rw) {
+                    jb = 0;
+                    if (++y == L) {
+                        y = 0;
+                        h ^= 1u;
+                    }
+                }
             }
             c0base += (uint32_t)((n + 3) / 4);
         }
