@@ -174,21 +174,28 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(BatchArgs b)
                     }
                     __syncthreads();
                     if (threadIdx.x == 0) {
+                        // blocks of 16 steps without a bound test per step,
+                        // then the remainder one by one
                         double x = s_x;
-                        for (int32_t j = 0; j < n; j += 16) {
+                        double *out = lnw + c0 + 1;
+                        int32_t j = 0;
+                        for (; j + 16 <= n; j += 16) {
                             double dv[16], bv[16];
 #pragma unroll
-                            for (int u = 0; u < 16; u++) {   // j + u < kChunk: kChunk % 16 == 0
+                            for (int u = 0; u < 16; u++) {
                                 dv[u] = s_ds[j + u];
                                 bv[u] = s_bn[j + u];
                             }
 #pragma unroll
                             for (int u = 0; u < 16; u++) {
-                                if (j + u < n) {
-                                    x = (dv[u] == dv[u]) ? x + dv[u] : bv[u];
-                                    lnw[c0 + j + u + 1] = x;
-                                }
+                                x = (dv[u] == dv[u]) ? x + dv[u] : bv[u];
+                                out[j + u] = x;
                             }
+                        }
+                        for (; j < n; j++) {
+                            const double dv = s_ds[j];
+                            x = (dv == dv) ? x + dv : s_bn[j];
+                            out[j] = x;
                         }
                         s_x = x;
                     }
