@@ -21,6 +21,8 @@ TEAM_SET = [(8, 16, 11, 40, 2.0, False), (16, 40, 12, 20, 2.0, True),
             (24, 20, 13, 10, 1.6, False), (8, 5, 14, 64, 2.5, True)]
 LANE_SET = [(6, 70, 21, 30, 2.0, False), (10, 261, 22, 12, 2.0, True),
             (16, 33, 23, 8, 1.7, False), (4, 64, 24, 50, 2.0, False)]
+# team tiles whose lattices do not fit a workgroup's LDS (lattices in HBM)
+HBM_SET = [(96, 32, 31, 4, 2.0, True), (128, 16, 32, 2, 2.0, False)]
 
 
 def _instances(spec):
@@ -104,6 +106,13 @@ def _run_against_oracle(orc, spec, tiles, use_graph, iters, stratified=False):
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_team_tiles_match_oracle(orc, use_graph):
     _run_against_oracle(orc, TEAM_SET, M.KERNEL_TEAM, use_graph, iters=4)
+
+
+def test_batch_team_tiles_lattices_in_hbm_match_oracle(orc):
+    """L = 96 and 128: the batch engine launches its build of the team kernel
+    for lattices in HBM (the next chunk's spins loaded during the current
+    chunk's chain), replayed from a graph."""
+    _run_against_oracle(orc, HBM_SET, M.KERNEL_TEAM, True, iters=2)
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
